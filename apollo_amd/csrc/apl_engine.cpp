@@ -1,0 +1,693 @@
+// apl_engine.cpp -- the C-ABI of the engine (include/apollo_dg.h).
+//
+// Host side: builds operators, geometry and connectivity (apl_host.cpp),
+// folds and permutes the operators into MFMA fragments, uploads everything
+// to the current HIP device and launches the kernels of apl_device.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/apollo_dg.h"
+#include "apl_device.h"
+#include "apl_host.h"
+
+namespace apl {
+template <int N>
+int launch_rhs(int eqn, int flux, int prec, hipStream_t s, const RhsArgs &a, int nblocks);
+}  // namespace apl
+
+using namespace apl;
+
+static thread_local std::string g_create_error;
+
+struct apl_handle_s {
+  apl_params params;
+  int K = 0, M = 0, Nv = 0;
+  Operators ops;
+  Geometry geo;
+  std::vector<int> etetf, fcode, vmapP;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int device = 0;
+  long long dev_bytes = 0;
+  std::string last_error;
+  // device buffers
+  double *d_geom = nullptr, *d_vert = nullptr, *d_I1D = nullptr;
+  int *d_fcode = nullptr, *d_fmask = nullptr;
+  void *d_A1 = nullptr, *d_A2 = nullptr, *d_A3 = nullptr;
+  unsigned *d_err = nullptr;
+  double *d_speed = nullptr, *d_t = nullptr;
+  double *d_qa = nullptr, *d_qb = nullptr, *d_res = nullptr, *d_out = nullptr;
+  // captured LSERK4 steps (even: A->B, odd: B->A)
+  hipGraphExec_t step_exec[2] = {nullptr, nullptr};
+  hipGraph_t step_graph[2] = {nullptr, nullptr};
+  const double *graph_qa = nullptr;
+  double graph_dt = 0.0;
+  // adjoint work buffers
+  double *d_adj[8] = {nullptr};
+  double *d_ckpt = nullptr;
+  long long ckpt_slots = 0;
+};
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+
+static int fail(apl_handle *h, int code, const std::string &msg) {
+  if (h) h->last_error = msg;
+  else g_create_error = msg;
+  return code;
+}
+
+#define HIPCHK(h, call)                                                            \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail(h, APL_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+static int dev_upload(apl_handle *h, T **dst, const T *src, size_t n) {
+  HIPCHK(h, hipMalloc((void **)dst, sizeof(T) * std::max<size_t>(n, 1)));
+  h->dev_bytes += (long long)(sizeof(T) * n);
+  if (n) HIPCHK(h, hipMemcpy(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice));
+  return APL_OK;
+}
+
+static int dev_alloc(apl_handle *h, void **dst, size_t bytes) {
+  HIPCHK(h, hipMalloc(dst, std::max<size_t>(bytes, 1)));
+  h->dev_bytes += (long long)bytes;
+  return APL_OK;
+}
+
+static int nvars(int eqn) {
+  switch (eqn) {
+    case APL_EQN_EULER5: return 5;
+    case APL_EQN_EULER4: return 4;
+    case APL_EQN_PHMAXWELL8: return 8;
+    case APL_EQN_MAXWELL_TM3: return 3;
+  }
+  return 0;
+}
+
+static int dispatch_rhs(apl_handle *h, const RhsArgs &a) {
+  const int nb = (h->K + 15) / 16;
+  const apl_params &p = h->params;
+  int rc;
+  switch (p.order) {
+    case 1: rc = launch_rhs<1>(p.eqn, p.flux, p.precision, h->stream, a, nb); break;
+    case 2: rc = launch_rhs<2>(p.eqn, p.flux, p.precision, h->stream, a, nb); break;
+    case 3: rc = launch_rhs<3>(p.eqn, p.flux, p.precision, h->stream, a, nb); break;
+    case 4: rc = launch_rhs<4>(p.eqn, p.flux, p.precision, h->stream, a, nb); break;
+    case 5: rc = launch_rhs<5>(p.eqn, p.flux, p.precision, h->stream, a, nb); break;
+    case 6: rc = launch_rhs<6>(p.eqn, p.flux, p.precision, h->stream, a, nb); break;
+    case 7: rc = launch_rhs<7>(p.eqn, p.flux, p.precision, h->stream, a, nb); break;
+    case 8: rc = launch_rhs<8>(p.eqn, p.flux, p.precision, h->stream, a, nb); break;
+    default: rc = APL_ERR_ORDER;
+  }
+  if (rc) return fail(h, rc, "RHS kernel launch failed");
+  return APL_OK;
+}
+
+static RhsArgs base_args(apl_handle *h) {
+  RhsArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.K = h->K;
+  a.geom = h->d_geom;
+  a.vert = h->d_vert;
+  a.fcode = h->d_fcode;
+  a.A1 = h->d_A1;
+  a.A2 = h->d_A2;
+  a.A3 = h->d_A3;
+  a.I1D = h->d_I1D;
+  a.fmask = h->d_fmask;
+  a.err = h->d_err;
+  const apl_params &p = h->params;
+  a.ph.gamma = p.gas_gamma;
+  a.ph.min_dens = p.min_density;
+  a.ph.min_pres = p.min_pressure;
+  a.ph.c0 = p.c0;
+  a.ph.chi = p.chi;
+  a.ph.gph = p.gamma_ph;
+  a.bc.n = p.n_bc;
+  for (int i = 0; i < APL_MAX_BC; ++i) {
+    a.bc.kind[i] = p.bc[i].kind;
+    for (int j = 0; j < 8; ++j) a.bc.p[i][j] = p.bc[i].p[j];
+  }
+  return a;
+}
+
+static int check_flags(apl_handle *h) {
+  unsigned ef = 0;
+  HIPCHK(h, hipMemcpyAsync(&ef, h->d_err, sizeof(unsigned), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (ef & EF_DENSITY) return fail(h, APL_ERR_NEG_DENSITY, "negative density in the Euler flux");
+  if (ef & EF_PRESSURE) return fail(h, APL_ERR_NEG_PRESSURE, "negative pressure in the Euler flux");
+  if (ef & EF_NAN) return fail(h, APL_ERR_NAN, "NaN in the RHS");
+  return APL_OK;
+}
+
+// Fold and permute the operators into MFMA fragments (see apl_device.h).
+template <typename T>
+static void make_fragments(const Operators &o, std::vector<T> &A1, std::vector<T> &A2,
+                           std::vector<T> &A3) {
+  const int Np = o.Np, Nc = o.Nc, Ng = o.Ng;
+  const int NKS1 = (Np + 3) / 4, NCH = (Nc + 15) / 16, NRT = (Np + 15) / 16, NKF = (3 * Ng + 3) / 4;
+  // A_r = V V^T Dr_c^T diag(w), A_s likewise, L = V V^T I_f^T diag(gw)
+  std::vector<double> Ar((size_t)Np * Nc, 0.0), As((size_t)Np * Nc, 0.0), L((size_t)Np * 3 * Ng, 0.0);
+  for (int n = 0; n < Np; ++n) {
+    for (int c = 0; c < Nc; ++c) {
+      double sr = 0.0, ss = 0.0;
+      for (int j = 0; j < Np; ++j) {
+        sr += o.VVT[(size_t)n * Np + j] * o.Drc[(size_t)c * Np + j];
+        ss += o.VVT[(size_t)n * Np + j] * o.Dsc[(size_t)c * Np + j];
+      }
+      Ar[(size_t)n * Nc + c] = sr * o.cw[c];
+      As[(size_t)n * Nc + c] = ss * o.cw[c];
+    }
+    for (int k = 0; k < 3 * Ng; ++k) {
+      double s = 0.0;
+      for (int j = 0; j < Np; ++j) s += o.VVT[(size_t)n * Np + j] * o.If[(size_t)k * Np + j];
+      L[(size_t)n * 3 * Ng + k] = s * o.gw[k % Ng];
+    }
+  }
+  A1.assign((size_t)NCH * NKS1 * 64, T(0));
+  for (int cc = 0; cc < NCH; ++cc)
+    for (int ks = 0; ks < NKS1; ++ks)
+      for (int l = 0; l < 64; ++l) {
+        const int c = 16 * cc + (l & 15), j = 4 * ks + (l >> 4);
+        if (c < Nc && j < Np) A1[((size_t)cc * NKS1 + ks) * 64 + l] = T(o.Vc[(size_t)c * Np + j]);
+      }
+  A2.assign((size_t)NRT * NCH * 64 * 8, T(0));
+  for (int rt = 0; rt < NRT; ++rt)
+    for (int cc = 0; cc < NCH; ++cc)
+      for (int l = 0; l < 64; ++l)
+        for (int r = 0; r < 4; ++r) {
+          const int n = 16 * rt + (l & 15), c = 16 * cc + acc_row<T>(l >> 4, r);
+          if (n < Np && c < Nc) {
+            const size_t b = (((size_t)rt * NCH + cc) * 64 + l) * 8;
+            A2[b + r] = T(Ar[(size_t)n * Nc + c]);
+            A2[b + 4 + r] = T(As[(size_t)n * Nc + c]);
+          }
+        }
+  A3.assign((size_t)NRT * NKF * 64, T(0));
+  for (int rt = 0; rt < NRT; ++rt)
+    for (int ks = 0; ks < NKF; ++ks)
+      for (int l = 0; l < 64; ++l) {
+        const int n = 16 * rt + (l & 15), k = 4 * ks + (l >> 4);
+        if (n < Np && k < 3 * Ng) A3[((size_t)rt * NKF + ks) * 64 + l] = T(-L[(size_t)n * 3 * Ng + k]);
+      }
+}
+
+template <typename T>
+static int upload_fragments(apl_handle *h) {
+  std::vector<T> A1, A2, A3;
+  make_fragments<T>(h->ops, A1, A2, A3);
+  int rc;
+  if ((rc = dev_upload<T>(h, (T **)&h->d_A1, A1.data(), A1.size()))) return rc;
+  if ((rc = dev_upload<T>(h, (T **)&h->d_A2, A2.data(), A2.size()))) return rc;
+  if ((rc = dev_upload<T>(h, (T **)&h->d_A3, A3.data(), A3.size()))) return rc;
+  return APL_OK;
+}
+
+static void free_graphs(apl_handle *h) {
+  for (int i = 0; i < 2; ++i) {
+    if (h->step_exec[i]) (void)hipGraphExecDestroy(h->step_exec[i]);
+    if (h->step_graph[i]) (void)hipGraphDestroy(h->step_graph[i]);
+    h->step_exec[i] = nullptr;
+    h->step_graph[i] = nullptr;
+  }
+  h->graph_qa = nullptr;
+}
+
+static void free_handle(apl_handle *h) {
+  if (!h) return;
+  free_graphs(h);
+  void *bufs[] = {h->d_geom, h->d_vert, h->d_I1D, h->d_fcode, h->d_fmask, h->d_A1, h->d_A2,
+                  h->d_A3, h->d_err, h->d_speed, h->d_t, h->d_qa, h->d_qb, h->d_res, h->d_out,
+                  h->d_ckpt};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  for (double *b : h->d_adj)
+    if (b) (void)hipFree(b);
+  if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+static size_t state_len(const apl_handle *h) { return (size_t)h->K * h->ops.Np * h->M; }
+
+static int ensure_work(apl_handle *h) {
+  const size_t bytes = sizeof(double) * state_len(h);
+  int rc;
+  if (!h->d_qa && (rc = dev_alloc(h, (void **)&h->d_qa, bytes))) return rc;
+  if (!h->d_qb && (rc = dev_alloc(h, (void **)&h->d_qb, bytes))) return rc;
+  if (!h->d_res && (rc = dev_alloc(h, (void **)&h->d_res, bytes))) return rc;
+  if (!h->d_out && (rc = dev_alloc(h, (void **)&h->d_out, bytes))) return rc;
+  return APL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+const char *apl_status_string(int s) {
+  switch (s) {
+    case APL_OK: return "ok";
+    case APL_ERR_ARG: return "invalid argument";
+    case APL_ERR_ORDER: return "polynomial order outside 1..8";
+    case APL_ERR_MESH_JACOBIAN: return "element with J <= 0";
+    case APL_ERR_MESH_EDGE: return "element with a zero-length edge";
+    case APL_ERR_MESH_TOPOLOGY: return "face shared by more than two elements";
+    case APL_ERR_HIP: return "HIP runtime error";
+    case APL_ERR_NAN: return "NaN in the RHS";
+    case APL_ERR_NEG_DENSITY: return "negative density";
+    case APL_ERR_NEG_PRESSURE: return "negative pressure";
+    case APL_ERR_ALLOC: return "allocation failed";
+    case APL_ERR_UNSUPPORTED: return "unsupported configuration";
+  }
+  return "unknown status";
+}
+
+const char *apl_last_error(const apl_handle *h) {
+  return h ? h->last_error.c_str() : g_create_error.c_str();
+}
+
+int apl_mesh_structured(int n, double x0, double x1, double y0, double y1, double jitter,
+                        unsigned long long seed, int periodic, double *VX, double *VY,
+                        int *EToV, int *vert_id, int *face_tag) {
+  if (n < 1 || !VX || !VY || !EToV || (periodic && n < 3) || !(x1 > x0) || !(y1 > y0) ||
+      jitter < 0.0 || jitter >= 0.5)
+    return APL_ERR_ARG;
+  structured_mesh(n, x0, x1, y0, y1, jitter, seed, periodic != 0, VX, VY, EToV, vert_id, face_tag);
+  return APL_OK;
+}
+
+int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out) {
+  g_create_error.clear();
+  if (!mesh || !params || !out) return fail(nullptr, APL_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (params->order < 1 || params->order > 8) return fail(nullptr, APL_ERR_ORDER, "order outside 1..8");
+  const int M = nvars(params->eqn);
+  if (!M) return fail(nullptr, APL_ERR_ARG, "unknown equation kind");
+  if (params->flux != APL_FLUX_LF &&
+      !(params->flux == APL_FLUX_UPWIND && params->eqn == APL_EQN_MAXWELL_TM3))
+    return fail(nullptr, APL_ERR_ARG, "upwind flux is available for MAXWELL_TM3 only");
+  if (params->precision != APL_PREC_FP64 && params->precision != APL_PREC_FP32_MFMA &&
+      params->precision != APL_PREC_BF16X3)
+    return fail(nullptr, APL_ERR_ARG, "unknown precision");
+  if (params->eqn == APL_EQN_MAXWELL_TM3 && (params->chi != 0.0 || params->gamma_ph != 0.0))
+    return fail(nullptr, APL_ERR_ARG, "MAXWELL_TM3 requires chi == gamma == 0");
+  if (params->n_bc < 0 || params->n_bc > APL_MAX_BC) return fail(nullptr, APL_ERR_ARG, "n_bc");
+  if (mesh->K < 1 || mesh->Nv < 3 || !mesh->VX || !mesh->VY || !mesh->EToV)
+    return fail(nullptr, APL_ERR_ARG, "empty or incomplete mesh");
+  for (long i = 0; i < 3L * mesh->K; ++i)
+    if (mesh->EToV[i] < 0 || mesh->EToV[i] >= mesh->Nv)
+      return fail(nullptr, APL_ERR_ARG, "EToV entry out of range");
+
+  apl_handle *h = new apl_handle();
+  h->params = *params;
+  h->K = mesh->K;
+  h->M = M;
+  h->Nv = mesh->Nv;
+  auto bail = [&](int rc) {
+    g_create_error = h->last_error;
+    free_handle(h);
+    return rc;
+  };
+  if (!build_operators(params->order, h->ops)) {
+    h->last_error = "operator construction failed";
+    return bail(APL_ERR_ORDER);
+  }
+  const Operators &o = h->ops;
+  int bad = -1;
+  int rc = build_geometry(o, h->K, mesh->VX, mesh->VY, mesh->EToV, h->geo, bad);
+  if (rc) {
+    h->last_error = "bad element " + std::to_string(bad);
+    return bail(rc);
+  }
+  rc = face_pair(h->K, mesh->Nv, mesh->EToV, mesh->vert_id, mesh->face_tag, h->etetf);
+  if (rc) {
+    h->last_error = "a face is shared by more than two elements";
+    return bail(rc);
+  }
+  // face codes and the vmapP index map
+  const int Np = o.Np, Nfp = o.Nfp;
+  h->fcode.resize((size_t)3 * h->K);
+  h->vmapP.assign((size_t)3 * h->K * Nfp, -1);
+  for (int k = 0; k < h->K; ++k)
+    for (int f = 0; f < 3; ++f) {
+      const int e2 = h->etetf[(size_t)6 * k + 2 * f], f2 = h->etetf[(size_t)6 * k + 2 * f + 1];
+      if (f2 >= 0) {
+        h->fcode[(size_t)3 * k + f] = e2 * 4 + f2;
+        for (int i = 0; i < Nfp; ++i)
+          h->vmapP[((size_t)3 * k + f) * Nfp + i] = e2 * Np + o.fmask[f2 * Nfp + (Nfp - 1 - i)];
+      } else {
+        // boundary condition index |tag| - 1 (wxnodaldg2dmethod.cc:418,569)
+        const int idx = std::abs(f2) - 1;
+        if (idx >= params->n_bc || params->bc[idx].kind == APL_BC_NONE) {
+          h->last_error = "boundary face of element " + std::to_string(k) +
+                          " has no boundary condition (tag " + std::to_string(std::abs(f2)) + ")";
+          return bail(APL_ERR_MESH_TOPOLOGY);
+        }
+        h->fcode[(size_t)3 * k + f] = -(idx + 1);
+      }
+    }
+  if (h->K > (1 << 29)) {
+    h->last_error = "K too large for 32-bit face codes";
+    return bail(APL_ERR_ARG);
+  }
+
+  // device side
+  hipError_t he = hipGetDevice(&h->device);
+  if (he != hipSuccess) {
+    h->last_error = std::string("hipGetDevice: ") + hipGetErrorString(he);
+    return bail(APL_ERR_HIP);
+  }
+  he = hipStreamCreate(&h->stream);
+  if (he != hipSuccess) {
+    h->last_error = std::string("hipStreamCreate: ") + hipGetErrorString(he);
+    return bail(APL_ERR_HIP);
+  }
+  h->own_stream = true;
+  std::vector<double> g((size_t)h->K * G_WORDS, 0.0);
+  for (int k = 0; k < h->K; ++k) {
+    const double *s = &h->geo.geom[(size_t)14 * k];
+    double *d = &g[(size_t)G_WORDS * k];
+    const double J = s[4];
+    d[G_JRX] = J * s[0];
+    d[G_JRY] = J * s[2];
+    d[G_JSX] = J * s[1];
+    d[G_JSY] = J * s[3];
+    d[G_INVJ] = 1.0 / J;
+    d[G_J] = J;
+    for (int f = 0; f < 3; ++f) {
+      d[G_NX + f] = s[5 + 3 * f];
+      d[G_NY + f] = s[6 + 3 * f];
+      d[G_SJ + f] = s[7 + 3 * f];
+    }
+  }
+  // face interpolation: I1D[f][g][i] = I_f[f Ng + g][Fmask[f][i]], then the
+  // Gauss abscissae
+  const int Ng = o.Ng;
+  std::vector<double> I1D((size_t)3 * Ng * Nfp + Ng);
+  for (int f = 0; f < 3; ++f)
+    for (int gq = 0; gq < Ng; ++gq)
+      for (int i = 0; i < Nfp; ++i)
+        I1D[((size_t)f * Ng + gq) * Nfp + i] = o.If[((size_t)f * Ng + gq) * Np + o.fmask[f * Nfp + i]];
+  for (int gq = 0; gq < Ng; ++gq) I1D[(size_t)3 * Ng * Nfp + gq] = o.gz[gq];
+
+  if ((rc = dev_upload<double>(h, &h->d_geom, g.data(), g.size()))) return bail(rc);
+  if ((rc = dev_upload<double>(h, &h->d_vert, h->geo.vert.data(), h->geo.vert.size()))) return bail(rc);
+  if ((rc = dev_upload<int>(h, &h->d_fcode, h->fcode.data(), h->fcode.size()))) return bail(rc);
+  if ((rc = dev_upload<int>(h, &h->d_fmask, o.fmask.data(), o.fmask.size()))) return bail(rc);
+  if ((rc = dev_upload<double>(h, &h->d_I1D, I1D.data(), I1D.size()))) return bail(rc);
+  if (params->precision == APL_PREC_FP64) rc = upload_fragments<double>(h);
+  else rc = upload_fragments<float>(h);
+  if (rc) return bail(rc);
+  if ((rc = dev_alloc(h, (void **)&h->d_err, sizeof(unsigned) * 4))) return bail(rc);
+  if ((rc = dev_alloc(h, (void **)&h->d_speed, sizeof(double) * ((h->K + 15) / 16)))) return bail(rc);
+  if ((rc = dev_alloc(h, (void **)&h->d_t, sizeof(double) * 2))) return bail(rc);
+  *out = h;
+  return APL_OK;
+}
+
+int apl_destroy(apl_handle *h) {
+  if (!h) return APL_ERR_ARG;
+  (void)hipStreamSynchronize(h->stream);
+  free_handle(h);
+  return APL_OK;
+}
+
+int apl_get_info(const apl_handle *h, apl_info *o) {
+  if (!h || !o) return APL_ERR_ARG;
+  o->K = h->K;
+  o->N = h->ops.N;
+  o->Np = h->ops.Np;
+  o->Nc = h->ops.Nc;
+  o->Ng = h->ops.Ng;
+  o->Nfp = h->ops.Nfp;
+  o->m = h->M;
+  o->dtscale = h->geo.dtscale;
+  o->rmin = h->ops.rmin;
+  o->device_bytes = h->dev_bytes;
+  return APL_OK;
+}
+
+int apl_set_stream(apl_handle *h, void *s) {
+  if (!h) return APL_ERR_ARG;
+  (void)hipStreamSynchronize(h->stream);
+  free_graphs(h);
+  if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+  h->own_stream = false;
+  h->stream = (hipStream_t)s;
+  return APL_OK;
+}
+
+int apl_get_index_maps(const apl_handle *h, int *etetf, int *fmask, int *vmapP) {
+  if (!h) return APL_ERR_ARG;
+  if (etetf) std::memcpy(etetf, h->etetf.data(), sizeof(int) * h->etetf.size());
+  if (fmask) std::memcpy(fmask, h->ops.fmask.data(), sizeof(int) * h->ops.fmask.size());
+  if (vmapP) std::memcpy(vmapP, h->vmapP.data(), sizeof(int) * h->vmapP.size());
+  return APL_OK;
+}
+
+int apl_get_geometry(const apl_handle *h, double *geom, double *x, double *y) {
+  if (!h) return APL_ERR_ARG;
+  if (geom) std::memcpy(geom, h->geo.geom.data(), sizeof(double) * h->geo.geom.size());
+  if (x) std::memcpy(x, h->geo.x.data(), sizeof(double) * h->geo.x.size());
+  if (y) std::memcpy(y, h->geo.y.data(), sizeof(double) * h->geo.y.size());
+  return APL_OK;
+}
+
+static int copy_operator(const Operators &o, int which, double *out, int capacity) {
+  const std::vector<double> *v = nullptr;
+  switch (which) {
+    case 0: v = &o.Vc; break;
+    case 1: v = &o.Drc; break;
+    case 2: v = &o.Dsc; break;
+    case 3: v = &o.If; break;
+    case 4: v = &o.VVT; break;
+    case 5: v = &o.cw; break;
+    case 6: v = &o.gz; break;
+    case 7: v = &o.gw; break;
+    case 8: v = &o.r; break;
+    case 9: v = &o.s; break;
+    default: return APL_ERR_ARG;
+  }
+  if ((size_t)capacity < v->size()) return APL_ERR_ARG;
+  std::memcpy(out, v->data(), sizeof(double) * v->size());
+  return APL_OK;
+}
+
+int apl_get_operator(const apl_handle *h, int which, double *out, int capacity) {
+  if (!h || !out) return APL_ERR_ARG;
+  return copy_operator(h->ops, which, out, capacity);
+}
+
+int apl_host_operator(int order, int which, double *out, int capacity) {
+  if (!out) return APL_ERR_ARG;
+  Operators o;
+  if (!build_operators(order, o)) return APL_ERR_ORDER;
+  return copy_operator(o, which, out, capacity);
+}
+
+int apl_host_connectivity(const apl_mesh *mesh, int *etetf) {
+  if (!mesh || !etetf || mesh->K < 1 || !mesh->EToV) return APL_ERR_ARG;
+  for (long i = 0; i < 3L * mesh->K; ++i)
+    if (mesh->EToV[i] < 0 || mesh->EToV[i] >= mesh->Nv) return APL_ERR_ARG;
+  std::vector<int> et;
+  const int rc = face_pair(mesh->K, mesh->Nv, mesh->EToV, mesh->vert_id, mesh->face_tag, et);
+  std::memcpy(etetf, et.data(), sizeof(int) * et.size());
+  return rc;
+}
+
+int apl_rhs(apl_handle *h, double t, const double *q, double *rhs, double cfl,
+            double *suggested_dt, int where) {
+  if (!h || !q || !rhs) return APL_ERR_ARG;
+  if (h->params.precision == APL_PREC_BF16X3)
+    return fail(h, APL_ERR_UNSUPPORTED, "BF16X3 RHS not built yet");
+  const size_t bytes = sizeof(double) * state_len(h);
+  int rc;
+  const double *dq = q;
+  double *drhs = rhs;
+  if (where == APL_HOST) {
+    if ((rc = ensure_work(h))) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->d_qa, q, bytes, hipMemcpyHostToDevice, h->stream));
+    dq = h->d_qa;
+    drhs = h->d_out;
+  }
+  HIPCHK(h, hipMemsetAsync(h->d_err, 0, sizeof(unsigned), h->stream));
+  RhsArgs a = base_args(h);
+  a.mode = 0;
+  a.q = dq;
+  a.out = drhs;
+  a.t = t;
+  a.speed_part = h->d_speed;
+  if ((rc = dispatch_rhs(h, a))) return rc;
+  if (where == APL_HOST)
+    HIPCHK(h, hipMemcpyAsync(rhs, drhs, bytes, hipMemcpyDeviceToHost, h->stream));
+  if ((rc = check_flags(h))) return rc;
+  if (suggested_dt) {
+    const int nb = (h->K + 15) / 16;
+    std::vector<double> sp(nb);
+    HIPCHK(h, hipMemcpy(sp.data(), h->d_speed, sizeof(double) * nb, hipMemcpyDeviceToHost));
+    double ms = 0.0;
+    for (double s : sp) ms = s > ms ? s : ms;
+    // wxnodaldg2dmethod.cc:523
+    *suggested_dt = 2. / 3. * cfl * h->geo.dtscale * (h->ops.rmin / ms);
+  }
+  return APL_OK;
+}
+
+int apl_initial_dt(apl_handle *h, const double *q, double cfl, double *dt, int where) {
+  if (!h || !q || !dt) return APL_ERR_ARG;
+  const size_t n = state_len(h);
+  std::vector<double> hq;
+  const double *qq = q;
+  if (where == APL_DEVICE) {
+    hq.resize(n);
+    HIPCHK(h, hipMemcpy(hq.data(), q, sizeof(double) * n, hipMemcpyDeviceToHost));
+    qq = hq.data();
+  }
+  // Roe speeds of the node state against itself (wxnodaldg2dmethod.cc:201-213,
+  // wxeulereqn.cc:95-234); phMaxwell's fastest wave is c0 max(chi, gamma, 1).
+  const apl_params &p = h->params;
+  const int M = h->M;
+  double maxSpeed = 0.0;
+  const bool euler = (p.eqn == APL_EQN_EULER5 || p.eqn == APL_EQN_EULER4);
+  for (size_t i = 0; i < n / M; ++i) {
+    const double *d = qq + i * M;
+    double lam = 0.0;
+    if (euler) {
+      const double rho = d[0], mu = d[1], mv = d[2];
+      const double mw = (p.eqn == APL_EQN_EULER5) ? d[3] : 0.0;
+      const double E = (p.eqn == APL_EQN_EULER5) ? d[4] : d[3];
+      const double g1 = p.gas_gamma - 1.0;
+      const double pr = g1 * (E - 0.5 * (mu * mu + mv * mv + mw * mw) / rho);
+      const double u = mu / rho, v = mv / rho, w = mw / rho;
+      const double enth = (E + pr) / rho;
+      const double a = std::sqrt(g1 * (enth - 0.5 * (u * u + v * v + w * w)));
+      for (double s : {u - a, u, u + a, v - a, v, v + a}) lam = std::max(lam, s * s);
+      lam = std::sqrt(lam);
+    } else {
+      lam = std::max(std::max(p.chi * p.c0, p.gamma_ph * p.c0), p.c0);
+    }
+    maxSpeed = std::max(maxSpeed, lam);
+  }
+  *dt = 2. / 3. * cfl * h->geo.dtscale * (h->ops.rmin / maxSpeed);
+  return APL_OK;
+}
+
+// LSERK4 coefficients (Carpenter & Kennedy 1994, five-stage fourth order).
+static const double rk4a[5] = {0.0, -567301805773.0 / 1357537059087.0,
+                               -2404267990393.0 / 2016746695238.0,
+                               -3550918686646.0 / 2091501179385.0,
+                               -1275806237668.0 / 842570457699.0};
+static const double rk4b[5] = {1432997174477.0 / 9575080441755.0,
+                               5161836677717.0 / 13612068292357.0,
+                               1720146321549.0 / 2090206949498.0,
+                               3134564353537.0 / 4481467310338.0,
+                               2277821191437.0 / 14882151754819.0};
+static const double rk4c[5] = {0.0, 1432997174477.0 / 9575080441755.0,
+                               2526269341429.0 / 6820363962896.0,
+                               2006345519317.0 / 3224310063776.0,
+                               2802321613138.0 / 2924317926251.0};
+
+// Enqueue the five stages of one step on h->stream; the state starts in
+// `src` and ends in `dst` (the other buffer).
+static int enqueue_step(apl_handle *h, double *src, double *dst, double dt, bool advance) {
+  RhsArgs a = base_args(h);
+  a.mode = 1;
+  a.res = h->d_res;
+  a.dt = dt;
+  a.tptr = h->d_t;
+  a.speed_part = nullptr;
+  double *bufs[2] = {src, dst};
+  for (int s = 0; s < 5; ++s) {
+    a.q = bufs[s & 1];
+    a.out = bufs[(s + 1) & 1];
+    a.rk_a = rk4a[s];
+    a.rk_b = rk4b[s];
+    a.tofs = rk4c[s] * dt;
+    int rc = dispatch_rhs(h, a);
+    if (rc) return rc;
+  }
+  if (advance) {
+    if (launch_advance_time(h->stream, h->d_t, dt)) return fail(h, APL_ERR_HIP, "time kernel");
+  }
+  return APL_OK;
+}
+
+static int build_graphs(apl_handle *h, double *qa, double *qb, double dt) {
+  free_graphs(h);
+  for (int i = 0; i < 2; ++i) {
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc = (i == 0) ? enqueue_step(h, qa, qb, dt, true) : enqueue_step(h, qb, qa, dt, true);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(h, APL_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+    h->step_graph[i] = g;
+    HIPCHK(h, hipGraphInstantiate(&h->step_exec[i], g, nullptr, nullptr, 0));
+  }
+  h->graph_qa = qa;
+  h->graph_dt = dt;
+  return APL_OK;
+}
+
+// Runs nsteps LSERK4 steps on device buffers qa (in/out) and qb (scratch).
+static int run_lserk4(apl_handle *h, double t0, double dt, int nsteps, double *qa, double *qb) {
+  int rc;
+  if (h->graph_qa != qa || h->graph_dt != dt || !h->step_exec[0])
+    if ((rc = build_graphs(h, qa, qb, dt))) return rc;
+  HIPCHK(h, hipMemcpyAsync(h->d_t, &t0, sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemsetAsync(h->d_err, 0, sizeof(unsigned), h->stream));
+  for (int s = 0; s < nsteps; ++s) HIPCHK(h, hipGraphLaunch(h->step_exec[s & 1], h->stream));
+  if (nsteps & 1)
+    HIPCHK(h, hipMemcpyAsync(qa, qb, sizeof(double) * state_len(h), hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return check_flags(h);
+}
+
+int apl_lserk4(apl_handle *h, double t0, double dt, int nsteps, double *q, int where) {
+  if (!h || !q || nsteps < 0) return APL_ERR_ARG;
+  if (h->params.precision == APL_PREC_BF16X3)
+    return fail(h, APL_ERR_UNSUPPORTED, "BF16X3 RHS not built yet");
+  if (nsteps == 0) return APL_OK;
+  int rc;
+  if ((rc = ensure_work(h))) return rc;
+  const size_t bytes = sizeof(double) * state_len(h);
+  if (where == APL_HOST) {
+    HIPCHK(h, hipMemcpyAsync(h->d_qa, q, bytes, hipMemcpyHostToDevice, h->stream));
+    if ((rc = run_lserk4(h, t0, dt, nsteps, h->d_qa, h->d_qb))) return rc;
+    HIPCHK(h, hipMemcpy(q, h->d_qa, bytes, hipMemcpyDeviceToHost));
+    return APL_OK;
+  }
+  return run_lserk4(h, t0, dt, nsteps, q, h->d_qb);
+}
+
+int apl_adjoint_rhs(apl_handle *h, double, const double *, const double *, double *, int) {
+  return h ? fail(h, APL_ERR_UNSUPPORTED, "adjoint not built yet") : APL_ERR_ARG;
+}
+
+int apl_lserk4_adjoint(apl_handle *h, double, double, int, int, const double *, const double *,
+                       double *, double *, int) {
+  return h ? fail(h, APL_ERR_UNSUPPORTED, "adjoint not built yet") : APL_ERR_ARG;
+}
+
+int apl_objective_disc_sq(apl_handle *h, const double *, int, double, double, double, double *,
+                          double *, int) {
+  return h ? fail(h, APL_ERR_UNSUPPORTED, "adjoint not built yet") : APL_ERR_ARG;
+}
+
+}  // extern "C"

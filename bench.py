@@ -1,0 +1,236 @@
+#!/usr/bin/env python3
+"""Benchmark: DG RHS DOF-updates/s on one MI355X.
+
+Default workload (Maxwell TM cavity, the configuration the headline metric
+is quoted on): 256 x 256 quads split into 2 triangles (K = 131072), vertices
+jittered +-0.1 h (seed 1), N = 5 (Np = 21), PEC walls with Apollo's
+transverse-magnetic boundary state, Ez = sin(3 pi x) sin(4 pi y), H = 0,
+LSERK4 with Apollo's dt formula at the deck CFL 0.5/(2N+1), fixed dt.
+
+A "step" is one LSERK4 step = 5 RHS evaluations fused with the stage
+updates, replayed as one hipGraph.  DOF-updates/s = K Np m x (RHS
+evaluations) / time with m = 3 (Ez, Bx, By).  `value` is the FP32-MFMA
+engine; the FP64 engine is timed the same way and reported beside it, with
+the L2 errors of both against the analytic cavity mode and of FP32 against
+FP64.  State lives in HBM before the timed region starts.
+
+    python bench.py [--gpus 1] [--steps 500] [--warmup 20] [--workload c2]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "DG RHS DOF-updates/s: FP64 plain vs FP32-MFMA and adjoint, at stated L2 error"
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15, "hbm": 8.0e12}
+
+
+def workload(name):
+    """Mesh + parameters + initial state of a benchmark workload."""
+    from apollo_amd import engine as E
+    if name == "c2":
+        n, N = 256, 5
+        mesh = E.structured_mesh(n, 0.0, 1.0, 0.0, 1.0, 0.1, 1, False)
+        params = dict(order=N, eqn=E.EQN_MAXWELL_TM3, bcs=[(E.BC_MAXWELL_TM, [])])
+        cfl = 0.5 / (2 * N + 1)
+        desc = ("2D Maxwell TM (Ez,Bx,By), unit square, PEC walls (Apollo TM boundary state), "
+                "256x256x2 jittered +-0.1h seed 1, K=131072, N=5, LF flux, LSERK4")
+        return dict(mesh=mesh, params=params, cfl=cfl, desc=desc, m=3, analytic="cavity")
+    if name == "c5":
+        n, N = 384, 8
+        mesh = E.structured_mesh(n, 0.0, 1.0, 0.0, 1.0, 0.1, 4, False)
+        params = dict(order=N, eqn=E.EQN_MAXWELL_TM3, bcs=[(E.BC_MAXWELL_TM, [])])
+        cfl = 0.5 / (2 * N + 1)
+        desc = "2D Maxwell TM, 384x384x2 jittered seed 4, K=294912, N=8, random 16-mode Ez"
+        return dict(mesh=mesh, params=params, cfl=cfl, desc=desc, m=3, analytic="fourier")
+    if name == "c3":
+        n, N = 512, 4
+        mesh = E.structured_mesh(n, -10.0, 10.0, -10.0, 10.0, 0.15, 2, True)
+        params = dict(order=N, eqn=E.EQN_EULER4, bcs=[])
+        desc = "2D Euler isentropic vortex, [-10,10]^2 periodic, 512x512x2 jittered 0.15h seed 2, K=524288, N=4"
+        return dict(mesh=mesh, params=params, cfl=0.5, desc=desc, m=4, analytic="vortex", L=20.0)
+    raise SystemExit("unknown workload %s" % name)
+
+
+def initial_state(w, x, y):
+    from apollo_amd import engine as E
+    if w["analytic"] == "cavity":
+        return E.tm_cavity_mode(x, y, 0.0, m=3)
+    if w["analytic"] == "fourier":
+        return E.random_fourier_ez(x, y, 16, 4, m=3)
+    return E.periodic_vortex_state(x, y, 0.0, w["L"], m=4)
+
+
+def exact_state(w, x, y, t):
+    from apollo_amd import engine as E
+    if w["analytic"] == "cavity":
+        return E.tm_cavity_mode(x, y, t, m=3)
+    if w["analytic"] == "vortex":
+        return E.periodic_vortex_state(x, y, t, w["L"], m=4)
+    return None
+
+
+def flop_alg(K, N, Np, Nc, Ng, m):
+    """Contraction flops of one RHS of Apollo's cubature form:
+    K 2 m (3 Nc Np + 6 Ng Np) (1 FMA = 2 flops)."""
+    return K * 2 * m * (3 * Nc * Np + 6 * Ng * Np)
+
+
+def bytes_alg_stage(K, Np, m):
+    """HBM bytes of one fused RHS + LSERK4 stage: read q and res, write q and
+    res (FP64), plus 136 B of geometry and connectivity per element."""
+    return K * (4 * Np * m * 8 + 136)
+
+
+def run_gpu(w, precision, steps, warmup, torch):
+    from apollo_amd import engine as E
+    p = dict(w["params"])
+    eng = E.Engine(w["mesh"], E.Params(precision=precision, **p))
+    stream = torch.cuda.Stream()
+    eng_stream = stream.cuda_stream
+    from apollo_amd.engine import lib
+    import ctypes
+    lib().apl_set_stream(eng._h, ctypes.c_void_p(eng_stream))
+    g, x, y = eng.geometry()
+    q0 = np.ascontiguousarray(initial_state(w, x, y))
+    dt = eng.suggested_dt(w["cfl"], 1.0) if w["analytic"] != "vortex" else None
+    if dt is None:
+        tmp = np.zeros_like(q0)
+        dt = eng.rhs(0.0, q0, tmp, cfl=w["cfl"])
+    with torch.cuda.stream(stream):
+        qd = torch.from_numpy(q0.copy()).cuda()
+        # warm-up (graph capture happens here)
+        eng.lserk4(0.0, dt, max(warmup, 1), qd)
+        qd.copy_(torch.from_numpy(q0))
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        eng.lserk4(0.0, dt, steps, qd)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    gpu_ms = ev0.elapsed_time(ev1)
+    qT = qd.cpu().numpy()
+    info = dict(K=eng.K, N=eng.N, Np=eng.Np, Nc=eng.Nc, Ng=eng.Ng, m=eng.m, dt=dt)
+    eng.close()
+    return dict(wall=wall, gpu_ms=gpu_ms, qT=qT, x=x, y=y, info=info, dt=dt)
+
+
+def cpu_baseline(w, nthreads, budget_s=15.0):
+    """Oracle RHS (Apollo's loop restated in C, Apollo's 8-variable Maxwell /
+    5-variable Euler layout) on the host cores, on the full mesh."""
+    from oracle import oracle as O
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import cases
+    from apollo_amd import engine as E
+    params = E.Params(**w["params"])
+    case = cases.Case(w["mesh"], params)
+    q = cases.to_oracle(params.eqn, initial_state(w, case.omesh.x, case.omesh.y))
+    O.rhs(case.omesh, case.ophys, 0.0, q, nthreads=nthreads)       # warm-up
+    t0 = time.perf_counter()
+    O.rhs(case.omesh, case.ophys, 0.0, q, nthreads=nthreads)
+    t1 = time.perf_counter() - t0
+    reps = int(max(3, min(20, budget_s / max(t1, 1e-6))))
+    for _ in range(2):
+        O.rhs(case.omesh, case.ophys, 0.0, q, nthreads=nthreads)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        O.rhs(case.omesh, case.ophys, 0.0, q, nthreads=nthreads)
+    el = time.perf_counter() - t0
+    dofs = case.mesh.K * case.ops.Np * w["m"]
+    return dict(value=dofs * reps / el, unit="DOF-updates/s", cores=nthreads, kind="port",
+                sample="%d RHS evaluations of the full mesh (K=%d, Apollo's %d-variable layout) "
+                       "after 3 warm-ups, OpenMP over elements" % (reps, case.mesh.K, q.shape[-1]),
+                rhs_per_s=reps / el)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", default="", help="fp64 or fp32: time one precision only")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the multi-GPU design is "
+              "documented, not built); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from apollo_amd import engine as E
+    assert torch.cuda.is_available()
+    w = workload(args.workload)
+    res = {}
+    for name, prec in (("fp64", E.PREC_FP64), ("fp32", E.PREC_FP32_MFMA)):
+        if args.only and args.only != name:
+            continue
+        print("bench: %s %s steps=%d" % (args.workload, name, args.steps), file=sys.stderr, flush=True)
+        res[name] = run_gpu(w, prec, args.steps, args.warmup, torch)
+    info = next(iter(res.values()))["info"]
+    K, N, Np, Nc, Ng, m = (info[k] for k in ("K", "N", "Np", "Nc", "Ng", "m"))
+    rhs_evals = 5 * args.steps
+    out = {}
+    for name, r in res.items():
+        ms = r["gpu_ms"]
+        kern_s = ms * 1e-3 / rhs_evals
+        dofs = K * Np * m * rhs_evals / (ms * 1e-3)
+        T = args.steps * r["dt"]
+        ex = exact_state(w, r["x"], r["y"], T)
+        err_exact = None
+        if ex is not None:
+            err_exact = float(np.linalg.norm((r["qT"] - ex).ravel()) / np.linalg.norm(ex.ravel()))
+        out[name] = dict(dof_updates_per_s=dofs, ms_per_step=ms / args.steps,
+                         kernel_us=kern_s * 1e6, l2_vs_analytic=err_exact, T=T,
+                         tflops_alg=flop_alg(K, N, Np, Nc, Ng, m) / kern_s / 1e12,
+                         gbs_alg=bytes_alg_stage(K, Np, m) / kern_s / 1e9)
+    head = "fp32" if "fp32" in out else "fp64"
+    l2_32_64 = None
+    if "fp32" in res and "fp64" in res:
+        a, b = res["fp32"]["qT"], res["fp64"]["qT"]
+        l2_32_64 = float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+    h = out[head]
+    peak = PEAK["fp32" if head == "fp32" else "fp64"]
+    achieved = h["tflops_alg"] * 1e12
+    line = {
+        "metric": METRIC,
+        "value": h["dof_updates_per_s"],
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": h["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if head == "fp32" else "f64",
+        "data": "synthetic",
+        "config": {"workload": w["desc"], "K": K, "N": N, "Np": Np, "Nc": Nc, "Ng": Ng,
+                   "m": m, "dt": info["dt"], "rhs_evals_per_step": 5,
+                   "precision": "FP32-MFMA contractions, FP64 state/geometry/stage update"
+                   if head == "fp32" else "FP64"},
+        "variants": out,
+        "l2_fp32_vs_fp64": l2_32_64,
+        "speedup_fp32_vs_fp64": (out["fp32"]["dof_updates_per_s"] / out["fp64"]["dof_updates_per_s"])
+        if ("fp32" in out and "fp64" in out) else None,
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                     "kernel": "rhs_kernel (fused RHS + LSERK4 stage), avg over %d launches" % rhs_evals,
+                     "flop_per_launch": flop_alg(K, N, Np, Nc, Ng, m)},
+    }
+    if not args.no_cpu_baseline:
+        nth = min(16, os.cpu_count() or 1)
+        print("bench: cpu baseline, %d threads" % nth, file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(w, nth)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,137 @@
+"""Shared test cases: meshes, parameters and states for the engine and the
+oracle, plus the maps between the engine's variable sets and Apollo's.
+
+The engine runs EULER4 / MAXWELL_TM3 (the physical 2D variables) as well as
+Apollo's own EULER5 / PHMAXWELL8 layouts; the oracle always uses Apollo's 5
+(Euler) or 8 (phMaxwell) variables.  EULER4 embeds with rho w = 0, TM3 embeds
+as (0, 0, Ez, Bx, By, 0, 0, 0)."""
+import os
+
+import numpy as np
+
+from apollo_amd import engine as E
+from oracle import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+EMBED = {E.EQN_EULER5: None, E.EQN_EULER4: [0, 1, 2, 4],
+         E.EQN_PHMAXWELL8: None, E.EQN_MAXWELL_TM3: [2, 3, 4]}
+
+
+def oracle_eqn(eqn):
+    return O.ORC_EULER if eqn in (E.EQN_EULER5, E.EQN_EULER4) else O.ORC_PHMAXWELL
+
+
+def to_oracle(eqn, q):
+    """Engine state [K,Np,m] -> oracle state [K,Np,5|8]."""
+    idx = EMBED[eqn]
+    if idx is None:
+        return np.ascontiguousarray(q)
+    mo = 5 if oracle_eqn(eqn) == O.ORC_EULER else 8
+    out = np.zeros(q.shape[:-1] + (mo,))
+    out[..., idx] = q
+    return out
+
+
+def from_oracle(eqn, qo):
+    idx = EMBED[eqn]
+    return np.ascontiguousarray(qo if idx is None else qo[..., idx])
+
+
+def vortex_bc(u0=1.0, v0=0.0, x0=0.0, y0=0.0, gamma=1.4, beta=5.0):
+    return (E.BC_ISENTROPIC_VORTEX, [u0, v0, x0, y0, gamma, beta])
+
+
+def oracle_phys(params):
+    eqn = oracle_eqn(params.eqn)
+    flux = O.ORC_FLUX_UPWIND if params.flux == E.FLUX_UPWIND else O.ORC_FLUX_LF
+    return O.phys(eqn, flux=flux, gamma=params.gas_gamma, min_dens=params.min_density,
+                  min_pres=params.min_pressure, c0=params.c0, chi=params.chi,
+                  gph=params.gamma_ph, bcs=params.bcs)
+
+
+class Case:
+    """A mesh + parameters, with the oracle's view of the same problem."""
+
+    def __init__(self, mesh, params, name=""):
+        self.mesh, self.params, self.name = mesh, params, name
+        self.ops = O.Ops(params.order)
+        self.omesh = O.Mesh(self.ops, mesh.VX, mesh.VY, mesh.EToV, mesh.vert_id, mesh.face_tag)
+        self.ophys = oracle_phys(params)
+
+    @property
+    def shape(self):
+        return (self.mesh.K, self.ops.Np, self.params.m)
+
+    def state(self, kind="smooth", seed=0, amp=1e-2):
+        """A valid state at the nodes (engine layout)."""
+        x, y = self.omesh.x, self.omesh.y
+        rng = np.random.default_rng(seed)
+        eqn, m = self.params.eqn, self.params.m
+        if eqn in (E.EQN_EULER4, E.EQN_EULER5):
+            q = E.vortex_state(x, y, 0.0, m=m)
+            q = q * (1.0 + amp * rng.standard_normal(q.shape))
+            if eqn == E.EQN_EULER5:
+                q[..., 3] = amp * rng.standard_normal(q.shape[:-1])
+            return np.ascontiguousarray(q)
+        if kind == "cavity":
+            return np.ascontiguousarray(E.tm_cavity_mode(x, y, 0.0, m=m))
+        return np.ascontiguousarray(rng.standard_normal(self.shape))
+
+    def oracle_rhs(self, q, t=0.0):
+        r, ms = O.rhs(self.omesh, self.ophys, t, to_oracle(self.params.eqn, q))
+        return from_oracle(self.params.eqn, r), ms
+
+
+def fp64_parity(case, got, q, t=0.0, tol=1e-12):
+    """FP64 parity bar.  The engine evaluates the geometric factors of each
+    straight-sided element exactly, Apollo recomputes them at every cubature
+    point from Dr_c x (wxcubature2d.cc:305), which adds rounding noise of order
+    N^2 |x| / h * eps.  The engine must be within `tol` of the oracle run with
+    exact geometric factors, and no farther from the faithful oracle than
+    max(tol, 2 x that noise).  Returns (err_exact, err_faithful, noise)."""
+    ref, _ = case.oracle_rhs(q, t)
+    with O.exact_geometry():
+        ref_exact, _ = case.oracle_rhs(q, t)
+    noise = rel_l2(ref, ref_exact)
+    e_exact, e_faith = rel_l2(got, ref_exact), rel_l2(got, ref)
+    print("%s: rel L2 vs exact-geometry oracle %.3e, vs faithful oracle %.3e "
+          "(oracle q8 noise %.3e)" % (case.name, e_exact, e_faith, noise))
+    assert e_exact <= tol, e_exact
+    assert e_faith <= max(tol, 2.0 * noise), (e_faith, noise)
+    return e_exact, e_faith, noise
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+# --- standard meshes ------------------------------------------------------
+
+def periodic_square(n, L=10.0, jitter=0.1, seed=0):
+    return E.structured_mesh(n, -L / 2, L / 2, -L / 2, L / 2, jitter, seed, True)
+
+
+def walled_unit_square(n, jitter=0.1, seed=1):
+    return E.structured_mesh(n, 0.0, 1.0, 0.0, 1.0, jitter, seed, False)
+
+
+def euler_case(n=16, N=3, m=4, periodic=True, precision=E.PREC_FP64, L=10.0, seed=0):
+    eqn = E.EQN_EULER4 if m == 4 else E.EQN_EULER5
+    if periodic:
+        mesh = periodic_square(n, L=L, seed=seed)
+        bcs = []
+    else:
+        mesh = E.structured_mesh(n, -L / 2, L / 2, -L / 2, L / 2, 0.1, seed, False)
+        bcs = [vortex_bc()]
+    return Case(mesh, E.Params(N, eqn, precision=precision, bcs=bcs), "euler")
+
+
+def maxwell_case(n=12, N=5, m=3, bc=E.BC_MAXWELL_TM, flux=E.FLUX_LF, precision=E.PREC_FP64,
+                 seed=1, chi=0.0, gph=0.0):
+    eqn = E.EQN_MAXWELL_TM3 if m == 3 else E.EQN_PHMAXWELL8
+    mesh = walled_unit_square(n, seed=seed)
+    return Case(mesh, E.Params(N, eqn, flux=flux, precision=precision, chi=chi, gamma_ph=gph,
+                               bcs=[(bc, [])]), "maxwell")

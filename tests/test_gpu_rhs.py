@@ -1,0 +1,161 @@
+"""GPU parity of the RHS and the stepper against the CPU oracle.
+
+Bar: FP64 fields within 1e-12 relative L2 of the oracle; integer maps
+bit-exact.  The FP32-MFMA variant is checked against the oracle with the
+tolerance stated in each test."""
+import numpy as np
+import pytest
+
+from apollo_amd import engine as E
+from oracle import oracle as O
+
+from cases import (Case, euler_case, fp64_parity, maxwell_case, rel_l2, vortex_bc,
+                   walled_unit_square)
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+
+def _rhs(case, q, t=0.0, precision=None):
+    params = case.params
+    if precision is not None:
+        params = E.Params(params.order, params.eqn, flux=params.flux, precision=precision,
+                          gas_gamma=params.gas_gamma, c0=params.c0, chi=params.chi,
+                          gamma_ph=params.gamma_ph, bcs=params.bcs)
+    eng = E.Engine(case.mesh, params)
+    out = np.zeros_like(q)
+    eng.rhs(t, q, out)
+    eng.close()
+    return out
+
+
+@pytest.mark.parametrize("m", [4, 5])
+def test_euler_periodic_rhs_fp64(m):
+    case = euler_case(n=16, N=3, m=m)
+    q = case.state()
+    fp64_parity(case, _rhs(case, q), q)
+
+
+def test_euler_vortex_bc_rhs_fp64():
+    case = euler_case(n=10, N=4, m=4, periodic=False)
+    q = case.state()
+    for t in (0.0, 0.37):
+        fp64_parity(case, _rhs(case, q, t), q, t)
+
+
+@pytest.mark.parametrize("bc", [E.BC_MAXWELL_TM, E.BC_PEC])
+@pytest.mark.parametrize("m", [3, 8])
+def test_maxwell_rhs_fp64(bc, m):
+    case = maxwell_case(n=10, N=5, m=m, bc=bc)
+    q = case.state()
+    fp64_parity(case, _rhs(case, q), q)
+
+
+def test_phmaxwell_chi_gamma_fp64():
+    case = maxwell_case(n=8, N=4, m=8, bc=E.BC_PEC, chi=1.3, gph=0.7)
+    q = case.state()
+    fp64_parity(case, _rhs(case, q), q)
+
+
+def test_tm_upwind_fp64():
+    case = maxwell_case(n=10, N=5, m=3, bc=E.BC_PEC, flux=E.FLUX_UPWIND)
+    q = case.state()
+    fp64_parity(case, _rhs(case, q), q)
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6, 7, 8])
+def test_all_orders_fp64(N):
+    for case in (euler_case(n=6, N=N, m=4), maxwell_case(n=5, N=N, m=3)):
+        q = case.state()
+        fp64_parity(case, _rhs(case, q), q)
+
+
+@pytest.mark.parametrize("N", [3, 5, 8])
+def test_fp32_mfma_rhs(N):
+    """FP32-MFMA contractions: relative L2 against the FP64 oracle <= 2e-5."""
+    for case in (euler_case(n=6, N=N, m=4), maxwell_case(n=5, N=N, m=3)):
+        q = case.state()
+        ref, _ = case.oracle_rhs(q)
+        err = rel_l2(_rhs(case, q, precision=E.PREC_FP32_MFMA), ref)
+        print("FP32 N=%d %s rel L2 %.3e" % (N, case.name, err))
+        assert err <= 2e-5
+
+
+def test_ragged_gmsh_mesh():
+    """250-triangle Gmsh mesh from Apollo's vortex deck: K not a multiple of 16."""
+    from apollo_amd.gmsh import read_gmsh
+    import os
+    from cases import GOLDEN
+    mesh = read_gmsh(os.path.join(GOLDEN, "meshes", "vortex_grid.msh"))
+    assert mesh.K % 16 != 0
+    params = E.Params(3, E.EQN_EULER5, bcs=[vortex_bc(x0=5.0)])
+    case = Case(mesh, params)
+    x, y = case.omesh.x, case.omesh.y
+    q = np.ascontiguousarray(E.vortex_state(x, y, 0.0, x0=5.0, m=5))
+    ref, ms = case.oracle_rhs(q, 0.1)
+    eng = E.Engine(mesh, params)
+    out = np.zeros_like(q)
+    dt = eng.rhs(0.1, q, out, cfl=0.2)
+    fp64_parity(case, out, q, 0.1)
+    assert abs(dt - O.suggested_dt(case.omesh, 0.2, ms)) <= 1e-14 * dt
+
+
+def test_index_maps_bit_exact():
+    case = maxwell_case(n=7, N=4, m=3)
+    eng = E.Engine(case.mesh, case.params)
+    et, fm, vp = eng.index_maps()
+    assert np.array_equal(et, case.omesh.etetf)
+    assert np.array_equal(fm, case.ops.fmask)
+    # vmapP: own face node i coincides with the neighbour node it names
+    g, x, y = eng.geometry()
+    K, Nfp = case.mesh.K, case.ops.Nfp
+    xf, yf = x.ravel(), y.ravel()
+    for k in range(K):
+        for f in range(3):
+            for i in range(Nfp):
+                j = vp[k, f, i]
+                own = k * case.ops.Np + fm[f, i]
+                if et[k, 2 * f + 1] < 0:
+                    assert j == -1
+                else:
+                    assert abs(xf[j] - xf[own]) < 1e-12 and abs(yf[j] - yf[own]) < 1e-12
+
+
+def test_lserk4_graph_matches_oracle():
+    case = euler_case(n=12, N=3, m=4)
+    q0 = case.state()
+    eng = E.Engine(case.mesh, case.params)
+    dt = 0.01
+    q = q0.copy()
+    eng.lserk4(0.0, dt, 7, q)
+    ref = O.lserk4(case.omesh, case.ophys, 0.0, dt, 7, __import__("cases").to_oracle(case.params.eqn, q0))
+    ref = __import__("cases").from_oracle(case.params.eqn, ref)
+    err = rel_l2(q - q0, ref - q0)
+    print("lserk4 increment rel L2 %.3e" % err)
+    assert err <= 1e-11
+    assert rel_l2(q, ref) <= 1e-13
+
+
+def test_lserk4_device_pointer_and_time_dependent_bc():
+    torch = pytest.importorskip("torch")
+    case = euler_case(n=8, N=3, m=4, periodic=False)
+    q0 = case.state()
+    eng = E.Engine(case.mesh, case.params)
+    qd = torch.from_numpy(q0.copy()).cuda()
+    eng.lserk4(0.25, 0.005, 6, qd)
+    from cases import to_oracle, from_oracle
+    ref = from_oracle(case.params.eqn, O.lserk4(case.omesh, case.ophys, 0.25, 0.005, 6,
+                                                to_oracle(case.params.eqn, q0)))
+    assert rel_l2(qd.cpu().numpy() - q0, ref - q0) <= 1e-11
+
+
+def test_rhs_error_codes():
+    """Apollo exits on rho <= 0 (wxeulereqn.cc:838-845); the engine returns a status."""
+    case = euler_case(n=4, N=2, m=4)
+    q = case.state()
+    q[3, 2, 0] = -1.0
+    eng = E.Engine(case.mesh, case.params)
+    with pytest.raises(E.ApolloError) as ei:
+        eng.rhs(0.0, q, np.zeros_like(q))
+    assert ei.value.status in (8, 9, 7)
