@@ -1,0 +1,188 @@
+"""The engine's C-ABI library and its host-side logic (CPU only: no compute
+call needs a GPU here)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from apollo_amd import engine as E
+from oracle import oracle as O
+
+from cases import GOLDEN
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "apollo_dg.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(apl_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_header_symbol():
+    L = E.lib()
+    names = header_functions()
+    assert len(names) >= 18
+    for name in names:
+        assert hasattr(L, name), name
+    assert set(names) == set(E.EXPORTS)
+
+
+def test_status_strings():
+    for s in range(12):
+        assert E.status_string(s)
+    assert E.status_string(0) == "ok"
+
+
+def test_create_validates_before_touching_the_device():
+    mesh = E.structured_mesh(3, 0, 1, 0, 1)
+    bad = [
+        (E.Params(9, E.EQN_EULER4), 2),                                      # order
+        (E.Params(3, 7), 1),                                                 # eqn
+        (E.Params(3, E.EQN_EULER4, flux=E.FLUX_UPWIND), 1),                  # upwind: TM only
+        (E.Params(3, E.EQN_MAXWELL_TM3, chi=1.0), 1),                        # TM needs chi=0
+    ]
+    for params, code in bad:
+        with pytest.raises(E.ApolloError) as ei:
+            E.Engine(mesh, params)
+        assert ei.value.status == code
+    # clockwise element -> J <= 0 (wxnodaldggeometry2d.cc:428-434)
+    cw = E.MeshData(mesh.VX, mesh.VY, mesh.EToV[:, ::-1].copy(), None, None)
+    with pytest.raises(E.ApolloError) as ei:
+        E.Engine(cw, E.Params(3, E.EQN_MAXWELL_TM3, bcs=[(E.BC_PEC, [])]))
+    assert ei.value.status == 3
+    # a boundary face whose tag has no boundary condition
+    with pytest.raises(E.ApolloError) as ei:
+        E.Engine(mesh, E.Params(3, E.EQN_MAXWELL_TM3))
+    assert ei.value.status == 5
+
+
+def test_host_operators_match_oracle():
+    for N in range(1, 9):
+        o = O.Ops(N)
+        for which, ref in ((0, o.Vc), (1, o.Drc), (2, o.Dsc), (3, o.If), (4, o.VVT), (5, o.cw),
+                           (8, o.r), (9, o.s)):
+            got = E.host_operator(N, which)
+            assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max()) * N, (N, which)
+        # the Gauss rule follows Apollo's gauleg bit for bit
+        assert np.array_equal(E.host_operator(N, 6), o.gz)
+        assert np.array_equal(E.host_operator(N, 7), o.gw)
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_host_connectivity_matches_oracle(periodic):
+    mesh = E.structured_mesh(9, -5, 5, -5, 5, 0.15, 3, periodic)
+    a = E.host_connectivity(mesh)
+    b = O.facepair(mesh.EToV, mesh.Nv, mesh.vert_id, mesh.face_tag)
+    assert np.array_equal(a, b)
+    if periodic:
+        assert (a >= 0).all()
+    else:
+        assert (a < 0).sum() == 2 * 4 * 9     # 36 boundary faces, two entries each
+
+
+@pytest.mark.parametrize("name", ["advection_test.msh", "small_square_4bc.msh",
+                                  "vortex_grid.msh", "tm_square.msh"])
+def test_host_connectivity_gmsh_fixtures(name):
+    from apollo_amd.gmsh import read_gmsh
+    mesh = read_gmsh(os.path.join(GOLDEN, "meshes", name))
+    assert np.array_equal(E.host_connectivity(mesh),
+                          O.facepair(mesh.EToV, mesh.Nv, None, mesh.face_tag))
+
+
+def test_gmsh_reader():
+    from apollo_amd.gmsh import read_gmsh
+    mesh = read_gmsh(os.path.join(GOLDEN, "meshes", "advection_test.msh"))
+    assert mesh.K == 16 and mesh.Nv == 13
+    assert sorted(set(mesh.face_tag.ravel()) - {0}) == [1, 2, 3, 4]
+    # every triangle of Apollo's example meshes is counter-clockwise
+    for name in ("advection_test.msh", "vortex_grid.msh", "tm_square.msh"):
+        m = read_gmsh(os.path.join(GOLDEN, "meshes", name))
+        x, y = m.VX[m.EToV], m.VY[m.EToV]
+        area = (x[:, 1] - x[:, 0]) * (y[:, 2] - y[:, 0]) - (x[:, 2] - x[:, 0]) * (y[:, 1] - y[:, 0])
+        assert (area > 0).all()
+
+
+# --- synthetic meshes -----------------------------------------------------------
+
+class MT19937_64:
+    """Reference MT19937-64 (Matsumoto & Nishimura), independent of libstdc++."""
+
+    def __init__(self, seed):
+        self.mt = [0] * 312
+        self.mt[0] = seed & (2 ** 64 - 1)
+        for i in range(1, 312):
+            prev = self.mt[i - 1]
+            self.mt[i] = (6364136223846793005 * (prev ^ (prev >> 62)) + i) & (2 ** 64 - 1)
+        self.idx = 312
+
+    def __call__(self):
+        if self.idx >= 312:
+            for i in range(312):
+                x = (self.mt[i] & 0xFFFFFFFF80000000) | (self.mt[(i + 1) % 312] & 0x7FFFFFFF)
+                xa = x >> 1
+                if x & 1:
+                    xa ^= 0xB5026F5AA96619E9
+                self.mt[i] = self.mt[(i + 156) % 312] ^ xa
+            self.idx = 0
+        x = self.mt[self.idx]
+        self.idx += 1
+        x ^= (x >> 29) & 0x5555555555555555
+        x ^= (x << 17) & 0x71D67FFFEDA60000
+        x ^= (x << 37) & 0xFFF7EEE000000000
+        x ^= x >> 43
+        return x & (2 ** 64 - 1)
+
+
+def test_mt19937_64_reference_value():
+    g = MT19937_64(5489)
+    for _ in range(9999):
+        g()
+    assert g() == 9981545732273789042   # required by the C++ standard [rand.predef]
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_structured_mesh_definition(periodic):
+    n, a, seed = 6, 0.15, 2
+    x0, x1, y0, y1 = -10.0, 10.0, -10.0, 10.0
+    mesh = E.structured_mesh(n, x0, x1, y0, y1, a, seed, periodic)
+    assert mesh.K == 2 * n * n and mesh.Nv == (n + 1) ** 2
+    g = MT19937_64(seed)
+    h = (x1 - x0) / n
+    VX = np.zeros(mesh.Nv)
+    VY = np.zeros(mesh.Nv)
+    for j in range(n + 1):
+        for i in range(n + 1):
+            x, y = x0 + i * h, y0 + j * h
+            if i == n:
+                x = x1
+            if j == n:
+                y = y1
+            if 0 < i < n and 0 < j < n:
+                x += (2.0 * (g() >> 11) * 2.0 ** -53 - 1.0) * a * h
+                y += (2.0 * (g() >> 11) * 2.0 ** -53 - 1.0) * a * h
+            VX[j * (n + 1) + i], VY[j * (n + 1) + i] = x, y
+    assert np.array_equal(mesh.VX, VX) and np.array_equal(mesh.VY, VY)
+    # element order: row-major quads, T0 = (v00, v10, v11), T1 = (v00, v11, v01)
+    q = 2 * n + 3
+    j, i = divmod(q, n)
+    v00 = j * (n + 1) + i
+    assert list(mesh.EToV[2 * q]) == [v00, v00 + 1, v00 + n + 2]
+    assert list(mesh.EToV[2 * q + 1]) == [v00, v00 + n + 2, v00 + n + 1]
+    xs, ys = mesh.VX[mesh.EToV], mesh.VY[mesh.EToV]
+    area = (xs[:, 1] - xs[:, 0]) * (ys[:, 2] - ys[:, 0]) - (xs[:, 2] - xs[:, 0]) * (ys[:, 1] - ys[:, 0])
+    assert (area > 0).all()
+    if periodic:
+        assert (mesh.face_tag == 0).all()
+        assert mesh.vert_id[n] == 0 and mesh.vert_id[n * (n + 1)] == 0
+    else:
+        assert (mesh.face_tag > 0).sum() == 4 * n
+
+
+def test_structured_mesh_rejects_bad_arguments():
+    with pytest.raises(E.ApolloError):
+        E.structured_mesh(2, 0, 1, 0, 1, 0.1, 0, True)      # periodic needs n >= 3
+    with pytest.raises(E.ApolloError):
+        E.structured_mesh(4, 0, 1, 0, 1, 0.6, 0, False)     # jitter must stay < 0.5
