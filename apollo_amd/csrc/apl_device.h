@@ -280,20 +280,141 @@ __device__ __forceinline__ void boundary_state(const BCTable &B, int idx, double
 }
 
 // ---------------------------------------------------------------------------
+// batch staging helpers (one wave, 16 elements of ES doubles each)
+// ---------------------------------------------------------------------------
+
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+// XT[w*16 + e] = q[(e0 + e)*ES + w]: all loads of a full batch are issued
+// before any LDS write (16-byte vector loads; the batch base is 128-byte
+// aligned because 16*ES*8 is a multiple of 128).
+template <int ES>
+__device__ __forceinline__ void stage_transposed(double *XT, const double *qb, int cnt, int lane) {
+  constexpr int NV2 = 16 * ES / 2;
+  constexpr int NIT = (NV2 + 63) / 64;
+  if (cnt == 16) {
+    dbl2 buf[NIT];
+#pragma unroll
+    for (int j = 0; j < NIT; ++j) {
+      const int idx = lane + 64 * j;
+      if (idx < NV2) buf[j] = ((const dbl2 *)qb)[idx];
+    }
+#pragma unroll
+    for (int j = 0; j < NIT; ++j) {
+      const int idx = lane + 64 * j;
+      if (idx < NV2) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int i = 2 * idx + h, e = i / ES, w = i - e * ES;
+          XT[w * 16 + e] = buf[j][h];
+        }
+      }
+    }
+  } else {
+    for (int i = lane; i < 16 * ES; i += 64) {
+      const int e = i / ES, w = i - e * ES;
+      XT[w * 16 + e] = (e < cnt) ? qb[i] : qb[w];
+    }
+  }
+}
+
+// Write the batch result OUT[e*ES + w] (LDS): rhs (mode 0) or the LSERK4
+// stage update res = a res + dt rhs, q_out = q_in + b res (mode 1).
+template <int ES>
+__device__ __forceinline__ unsigned write_batch(const RhsArgs &a, const double *OUT, long e0,
+                                                int cnt, int lane) {
+  constexpr int NV2 = 16 * ES / 2;
+  constexpr int NIT = (NV2 + 63) / 64;
+  const long base = e0 * ES;
+  unsigned ef = 0;
+  if (cnt == 16) {
+    if (a.mode == 0) {
+#pragma unroll
+      for (int j = 0; j < NIT; ++j) {
+        const int idx = lane + 64 * j;
+        if (idx < NV2) {
+          const dbl2 v = dbl2{OUT[2 * idx], OUT[2 * idx + 1]};
+          if (v[0] != v[0] || v[1] != v[1]) ef |= EF_NAN;
+          ((dbl2 *)(a.out + base))[idx] = v;
+        }
+      }
+    } else {
+      dbl2 rb[NIT], qv[NIT];
+#pragma unroll
+      for (int j = 0; j < NIT; ++j) {
+        const int idx = lane + 64 * j;
+        if (idx < NV2) {
+          if (a.rk_a != 0.0) rb[j] = ((const dbl2 *)(a.res + base))[idx];
+          qv[j] = ((const dbl2 *)(a.q + base))[idx];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < NIT; ++j) {
+        const int idx = lane + 64 * j;
+        if (idx < NV2) {
+          dbl2 r;
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const double val = OUT[2 * idx + h];
+            if (val != val) ef |= EF_NAN;
+            r[h] = (a.rk_a == 0.0) ? a.dt * val : a.rk_a * rb[j][h] + a.dt * val;
+          }
+          ((dbl2 *)(a.res + base))[idx] = r;
+          ((dbl2 *)(a.out + base))[idx] = qv[j] + a.rk_b * r;
+        }
+      }
+    }
+  } else {
+    for (int i = lane; i < cnt * ES; i += 64) {
+      const double val = OUT[i];
+      if (val != val) ef |= EF_NAN;
+      if (a.mode == 0) {
+        a.out[base + i] = val;
+      } else {
+        const double r = (a.rk_a == 0.0) ? a.dt * val : a.rk_a * a.res[base + i] + a.dt * val;
+        a.res[base + i] = r;
+        a.out[base + i] = a.q[base + i] + a.rk_b * r;
+      }
+    }
+  }
+  return ef;
+}
+
+// ---------------------------------------------------------------------------
 // the fused RHS kernel
 // ---------------------------------------------------------------------------
 
+// LDS of one single-wave workgroup (doubles).  The batch state is staged
+// transposed, XT[(node*M + v)*16 + e], so that B-operand and face-node reads
+// of the 16 lanes of a lane group hit consecutive addresses; the neighbours'
+// face nodes are staged as PT[((f*Nfp + i)*M + v)*16 + e] in the own face's
+// node order (neighbour node Fmask[nf][Nfp-1-i] for own node i), and the own
+// face nodes likewise as QT (node Fmask[f][i]).  The small tables
+// I1D[3][Ng][Nfp] and Fmask[3][Nfp] follow.
 template <int N, int EQN>
 struct Lds {
   static constexpr int M = Vars<EQN>::M;
-  static constexpr int X = 16 * Order<N>::Np * M;          // own state
-  static constexpr int QP = 16 * 3 * Order<N>::Nfp * M;    // neighbour face nodes
-  static constexpr int doubles = X + QP;
+  static constexpr int Np = Order<N>::Np, Nfp = Order<N>::Nfp, Ng = Order<N>::Ng;
+  static constexpr int XT = Order<N>::NKS1 * 4 * M * 16;   // rows >= Np are zero
+  static constexpr int PT = 3 * Nfp * M * 16;
+  static constexpr int I1D = 3 * Ng * Nfp;
+  static constexpr int FM = (3 * Nfp + 1) / 2;   // ints packed into doubles
+  static constexpr int doubles = XT + 2 * PT + I1D + FM;
+};
+
+// Waves per SIMD the register allocator is asked to allow: two where the
+// kernel fits 256 VGPRs without spilling (measured with
+// -Rpass-analysis=kernel-resource-usage), one otherwise.
+template <int N, int EQN>
+struct MinWaves {
+  static constexpr int value =
+      (EQN == APL_EQN_PHMAXWELL8 || N >= 6) ? 1 : (N == 5 ? (EQN == APL_EQN_MAXWELL_TM3 ? 2 : 1) : 2);
 };
 
 template <int N, int EQN, int FLUX, typename T>
-__global__ void __launch_bounds__(64) rhs_kernel(RhsArgs a) {
+__global__ void __launch_bounds__(64, (MinWaves<N, EQN>::value)) rhs_kernel(RhsArgs a) {
   using O = Order<N>;
+  using L = Lds<N, EQN>;
   constexpr int M = Vars<EQN>::M;
   constexpr int Np = O::Np, Nc = O::Nc, Ng = O::Ng, Nfp = O::Nfp;
   constexpr int NKS1 = O::NKS1, NCH = O::NCH, NRT = O::NRT, NKF = O::NKF;
@@ -301,38 +422,64 @@ __global__ void __launch_bounds__(64) rhs_kernel(RhsArgs a) {
   typedef typename Acc<T>::type V4;
 
   extern __shared__ double lds[];
-  double *X = lds;                       // [16][Np][M]
-  double *QPn = lds + Lds<N, EQN>::X;    // [16][3][Nfp][M]
+  double *XT = lds;
+  double *PT = lds + L::XT;
+  double *QT = PT + L::PT;
+  double *I1 = QT + L::PT;
+  int *FM = (int *)(I1 + L::I1D);
 
   const int lane = threadIdx.x;
   const int el = lane & 15, lg = lane >> 4;
   const long e0 = (long)blockIdx.x * 16;
   const int cnt = (int)min((long)16, (long)a.K - e0);
 
-  // 1. stage the batch's state (coalesced; missing elements repeat element 0)
-  const double *qb = a.q + e0 * ES;
-  for (int i = lane; i < 16 * ES; i += 64) X[i] = (i < cnt * ES) ? qb[i] : qb[i % ES];
-
-  // 2. gather the neighbours' face nodes
-  for (int it = lane; it < 16 * 3 * Nfp; it += 64) {
-    const int e = it / (3 * Nfp), rem = it % (3 * Nfp), f = rem / Nfp, i = rem % Nfp;
-    const long ge = e0 + (e < cnt ? e : 0);
-    const int code = a.fcode[ge * 3 + f];
-    double *dst = QPn + ((e * 3 + f) * Nfp + i) * M;
-    if (code >= 0) {
-      const long nb = code >> 2;
-      const int nf = code & 3;
-      const double *src = a.q + (nb * Np + a.fmask[nf * Nfp + i]) * M;
-#pragma unroll
-      for (int v = 0; v < M; ++v) dst[v] = src[v];
-    }
-  }
-  __syncthreads();
-
-  // 3. geometry of this lane's element
+  // element data of this lane (element el; missing elements repeat element 0)
   const long ge = e0 + (el < cnt ? el : 0);
   const double *gk = a.geom + ge * G_WORDS;
   const double jrx = gk[G_JRX], jry = gk[G_JRY], jsx = gk[G_JSX], jsy = gk[G_JSY];
+  const double Jel = gk[G_J];
+  const double nx0 = gk[G_NX], nx1 = gk[G_NX + 1], nx2 = gk[G_NX + 2];
+  const double ny0 = gk[G_NY], ny1 = gk[G_NY + 1], ny2 = gk[G_NY + 2];
+  const double sj0 = gk[G_SJ], sj1 = gk[G_SJ + 1], sj2 = gk[G_SJ + 2];
+  const int code0 = a.fcode[ge * 3], code1 = a.fcode[ge * 3 + 1], code2 = a.fcode[ge * 3 + 2];
+
+  // 1. small tables
+  for (int i = lane; i < L::I1D; i += 64) I1[i] = a.I1D[i];
+  for (int i = lane; i < 3 * Nfp; i += 64) FM[i] = a.fmask[i];
+
+  // 2. stage the batch state, transposed (coalesced global reads)
+  stage_transposed<ES>(XT, a.q + e0 * ES, cnt, lane);
+  for (int i = 16 * ES + lane; i < L::XT; i += 64) XT[i] = 0.0;
+  __syncthreads();
+
+  // 3. gather the neighbours' face nodes; item (k = f*Nfp + i, e), e fastest
+  constexpr int NGI = (16 * 3 * Nfp + 63) / 64;
+#pragma unroll
+  for (int j = 0; j < NGI; ++j) {
+    const int it = lane + 64 * j;
+    if (it < 16 * 3 * Nfp) {
+      const int e = it & 15, k = it >> 4, f = k / Nfp, i = k - f * Nfp;
+      const long gee = e0 + (e < cnt ? e : 0);
+      const int code = a.fcode[gee * 3 + f];
+      double v8[M];
+      if (code >= 0) {
+        const long nb = code >> 2;
+        const int nf = code & 3;
+        const double *src = a.q + (nb * Np + FM[nf * Nfp + (Nfp - 1 - i)]) * M;
+#pragma unroll
+        for (int v = 0; v < M; ++v) v8[v] = src[v];
+      } else {
+#pragma unroll
+        for (int v = 0; v < M; ++v) v8[v] = 0.0;
+      }
+#pragma unroll
+      for (int v = 0; v < M; ++v) {
+        PT[(k * M + v) * 16 + e] = v8[v];
+        QT[(k * M + v) * 16 + e] = XT[(FM[f * Nfp + i] * M + v) * 16 + e];
+      }
+    }
+  }
+  __syncthreads();
 
   V4 acc[NRT][M];
 #pragma unroll
@@ -344,28 +491,49 @@ __global__ void __launch_bounds__(64) rhs_kernel(RhsArgs a) {
   const T *A1 = (const T *)a.A1;
   const T *A2 = (const T *)a.A2;
   const T *A3 = (const T *)a.A3;
-  const double *Xe = X + el * ES;
 
-  // 4. volume: chunks of 16 cubature points
+  // 4. volume: chunks of 16 cubature points.  The A_r|A_s fragments of the
+  // next chunk are loaded while the current one computes; GEMM1's operands
+  // are loaded in one batch at the top of each chunk.
+  T a2n[NRT][8];
+#pragma unroll
+  for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a2n[rt][j] = A2[((size_t)(rt * NCH) * 64 + lane) * 8 + j];
+
 #pragma unroll 1
   for (int cc = 0; cc < NCH; ++cc) {
+    T a1[NKS1], b1[NKS1][M], a2[NRT][8];
+#pragma unroll
+    for (int ks = 0; ks < NKS1; ++ks) {
+      a1[ks] = A1[(cc * NKS1 + ks) * 64 + lane];
+#pragma unroll
+      for (int v = 0; v < M; ++v) b1[ks][v] = T(XT[((4 * ks + lg) * M + v) * 16 + el]);
+    }
+#pragma unroll
+    for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) a2[rt][j] = a2n[rt][j];
+    if (cc + 1 < NCH) {
+#pragma unroll
+      for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          a2n[rt][j] = A2[((size_t)(rt * NCH + cc + 1) * 64 + lane) * 8 + j];
+    }
     V4 qc[M];
 #pragma unroll
     for (int v = 0; v < M; ++v) qc[v] = V4{0, 0, 0, 0};
 #pragma unroll
-    for (int ks = 0; ks < NKS1; ++ks) {
-      const T av = A1[(cc * NKS1 + ks) * 64 + lane];
-      const int node = 4 * ks + lg;
+    for (int ks = 0; ks < NKS1; ++ks)
 #pragma unroll
-      for (int v = 0; v < M; ++v) {
-        const T bv = (node < Np) ? T(Xe[node * M + v]) : T(0);
-        qc[v] = mma(av, bv, qc[v]);
-      }
-    }
-    T phr[M][4], phs[M][4];
+      for (int v = 0; v < M; ++v) qc[v] = mma(a1[ks], b1[ks][v], qc[v]);
+    // cubature point r of this lane: flux, geometric scaling, and its two
+    // rank-4 updates of the output tiles
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int c = 16 * cc + acc_row<T>(lg, r);
+      T phr[M], phs[M];
       if (c < Nc) {
         T qv[M], F[M], G[M];
 #pragma unroll
@@ -373,26 +541,19 @@ __global__ void __launch_bounds__(64) rhs_kernel(RhsArgs a) {
         ef |= phys_flux<EQN, T>(a.ph, qv, F, G);
 #pragma unroll
         for (int v = 0; v < M; ++v) {
-          phr[v][r] = T(jrx * double(F[v]) + jry * double(G[v]));
-          phs[v][r] = T(jsx * double(F[v]) + jsy * double(G[v]));
+          phr[v] = T(jrx * double(F[v]) + jry * double(G[v]));
+          phs[v] = T(jsx * double(F[v]) + jsy * double(G[v]));
         }
       } else {
 #pragma unroll
-        for (int v = 0; v < M; ++v) phr[v][r] = phs[v][r] = T(0);
+        for (int v = 0; v < M; ++v) phr[v] = phs[v] = T(0);
       }
-    }
 #pragma unroll
-    for (int rt = 0; rt < NRT; ++rt) {
-      const T *frag = A2 + ((size_t)(rt * NCH + cc) * 64 + lane) * 8;
-      T a2[8];
+      for (int rt = 0; rt < NRT; ++rt)
 #pragma unroll
-      for (int j = 0; j < 8; ++j) a2[j] = frag[j];
-#pragma unroll
-      for (int v = 0; v < M; ++v)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          acc[rt][v] = mma(a2[r], phr[v][r], acc[rt][v]);
-          acc[rt][v] = mma(a2[4 + r], phs[v][r], acc[rt][v]);
+        for (int v = 0; v < M; ++v) {
+          acc[rt][v] = mma(a2[rt][r], phr[v], acc[rt][v]);
+          acc[rt][v] = mma(a2[rt][4 + r], phs[v], acc[rt][v]);
         }
     }
   }
@@ -403,40 +564,40 @@ __global__ void __launch_bounds__(64) rhs_kernel(RhsArgs a) {
 #pragma unroll 1
   for (int ks = 0; ks < NKF; ++ks) {
     const int kf = 4 * ks + lg;
+    T a3[NRT];
+#pragma unroll
+    for (int rt = 0; rt < NRT; ++rt) a3[rt] = A3[(rt * NKF + ks) * 64 + lane];
     T fsT[M];
     if (kf < 3 * Ng) {
-      const int f = kf / Ng, g = kf % Ng;
+      const int f = kf / Ng, g = kf - f * Ng;
+      const int code = f == 0 ? code0 : (f == 1 ? code1 : code2);
+      const double nx = f == 0 ? nx0 : (f == 1 ? nx1 : nx2);
+      const double ny = f == 0 ? ny0 : (f == 1 ? ny1 : ny2);
+      const double sj = f == 0 ? sj0 : (f == 1 ? sj1 : sj2);
       double qM[M], qP[M], fs[M];
 #pragma unroll
-      for (int v = 0; v < M; ++v) qM[v] = 0.0;
-      const double *I = a.I1D + (f * Ng + g) * Nfp;
+      for (int v = 0; v < M; ++v) qM[v] = qP[v] = 0.0;
+      const double *Io = I1 + (f * Ng + g) * Nfp;
 #pragma unroll
       for (int i = 0; i < Nfp; ++i) {
-        const double w = I[i];
-        const double *src = Xe + a.fmask[f * Nfp + i] * M;
+        const double w = Io[i];
 #pragma unroll
-        for (int v = 0; v < M; ++v) qM[v] += w * src[v];
+        for (int v = 0; v < M; ++v) qM[v] += w * QT[((f * Nfp + i) * M + v) * 16 + el];
       }
-      const int code = a.fcode[ge * 3 + f];
-      const double nx = gk[G_NX + f], ny = gk[G_NY + f];
       if (code >= 0) {
+        // own node i <-> neighbour node Fmask[nf][Nfp-1-i]; the neighbour's
+        // trace at its Gauss point Ng-1-g is sum_j I1D[nf][Ng-1-g][j] q[Fmask[nf][j]]
         const int nf = code & 3;
-        const double *In = a.I1D + (nf * Ng + (Ng - 1 - g)) * Nfp;
-        const double *src = QPn + (el * 3 + f) * Nfp * M;
-#pragma unroll
-        for (int v = 0; v < M; ++v) qP[v] = 0.0;
+        const double *In = I1 + (nf * Ng + (Ng - 1 - g)) * Nfp;
 #pragma unroll
         for (int i = 0; i < Nfp; ++i) {
-          const double w = In[i];
+          const double w = In[Nfp - 1 - i];
 #pragma unroll
-          for (int v = 0; v < M; ++v) qP[v] += w * src[i * M + v];
+          for (int v = 0; v < M; ++v) qP[v] += w * PT[((f * Nfp + i) * M + v) * 16 + el];
         }
       } else {
-        // Gauss point coordinates on the straight edge
         const double *vk = a.vert + ge * 6;
         const int va = f, vb = (f + 1) % 3;
-        // I1D rows carry the 1D abscissa implicitly; recover z from the
-        // Gauss rule stored after I1D
         const double z = a.I1D[3 * Ng * Nfp + g];
         const double s = 0.5 * (1.0 + z);
         const double xg = vk[2 * va] + s * (vk[2 * vb] - vk[2 * va]);
@@ -446,7 +607,6 @@ __global__ void __launch_bounds__(64) rhs_kernel(RhsArgs a) {
       double lam;
       ef |= num_flux<EQN, FLUX>(a.ph, nx, ny, qM, qP, fs, &lam);
       lmax = lam > lmax ? lam : lmax;
-      const double sj = gk[G_SJ + f];
 #pragma unroll
       for (int v = 0; v < M; ++v) fsT[v] = T(fs[v] * sj);
     } else {
@@ -454,16 +614,14 @@ __global__ void __launch_bounds__(64) rhs_kernel(RhsArgs a) {
       for (int v = 0; v < M; ++v) fsT[v] = T(0);
     }
 #pragma unroll
-    for (int rt = 0; rt < NRT; ++rt) {
-      const T a3 = A3[(rt * NKF + ks) * 64 + lane];
+    for (int rt = 0; rt < NRT; ++rt)
 #pragma unroll
-      for (int v = 0; v < M; ++v) acc[rt][v] = mma(a3, fsT[v], acc[rt][v]);
-    }
+      for (int v = 0; v < M; ++v) acc[rt][v] = mma(a3[rt], fsT[v], acc[rt][v]);
   }
 
-  // 6. epilogue: / J, stage through LDS, coalesced write
+  // 6. epilogue: / J, stage through LDS as [e][n][v], coalesced write
   __syncthreads();
-  const double Jel = gk[G_J];
+  double *OUT = lds;
 #pragma unroll
   for (int rt = 0; rt < NRT; ++rt)
 #pragma unroll
@@ -471,27 +629,216 @@ __global__ void __launch_bounds__(64) rhs_kernel(RhsArgs a) {
       const int n = 16 * rt + acc_row<T>(lg, r);
       if (n < Np) {
 #pragma unroll
-        for (int v = 0; v < M; ++v) X[el * ES + n * M + v] = double(acc[rt][v][r]) / Jel;
+        for (int v = 0; v < M; ++v) OUT[el * ES + n * M + v] = double(acc[rt][v][r]) / Jel;
       }
     }
   __syncthreads();
-  const long base = e0 * ES;
-  if (a.mode == 0) {
-    for (int i = lane; i < cnt * ES; i += 64) {
-      const double val = X[i];
-      if (val != val) ef |= EF_NAN;
-      a.out[base + i] = val;
-    }
-  } else {
-    for (int i = lane; i < cnt * ES; i += 64) {
-      const double val = X[i];
-      if (val != val) ef |= EF_NAN;
-      const double r = (a.rk_a == 0.0) ? a.dt * val : a.rk_a * a.res[base + i] + a.dt * val;
-      a.res[base + i] = r;
-      a.out[base + i] = a.q[base + i] + a.rk_b * r;
+  ef |= write_batch<ES>(a, OUT, e0, cnt, lane);
+  // wave-wide max speed and flags
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(lmax, off);
+    lmax = o > lmax ? o : lmax;
+    ef |= __shfl_xor(ef, off);
+  }
+  if (lane == 0) {
+    if (a.speed_part) a.speed_part[blockIdx.x] = lmax;
+    if (ef) atomicOr(a.err, ef);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// the nodal (linear-flux) RHS kernel
+// ---------------------------------------------------------------------------
+//
+// For a linear flux F(q) = Ax q, G(q) = Ay q the cubature volume term is
+// A_r Phi_r + A_s Phi_s with Phi_r = (jrx Ax + jry Ay)(V_c Q), which equals
+// Dt_r Q (jrx Ax + jry Ay)^T with Dt_r = A_r V_c (Np x Np), and the face term
+// L S(Gauss points) equals Lt S(face nodes) with Lt = L blkdiag(I1D) because
+// F* is linear in (q-, q+) and the boundary states are linear reflections.
+// Same result to rounding, 4-5x fewer MFMAs:
+//   P_r = Dt_r Q, P_s = Dt_s Q        GEMM   (Np x Np)(Np x 16) per var
+//   R   = jrx F(P_r) + jry G(P_r) + jsx F(P_s) + jsy G(P_s)   at the nodes
+//   R  -= Lt S,  S = F*(q-, q+) L/2 at the 3(N+1) face nodes
+//   rhs = R / J
+
+template <int N, int EQN>
+struct LinLds {
+  static constexpr int M = Vars<EQN>::M;
+  static constexpr int Nfp = Order<N>::Nfp;
+  static constexpr int XT = Order<N>::NKS1 * 4 * M * 16;
+  static constexpr int PT = 3 * Nfp * M * 16;
+  static constexpr int FM = (3 * Nfp + 1) / 2;
+  static constexpr int doubles = XT + 2 * PT + FM;
+};
+
+template <int N>
+struct LinOrder {
+  static constexpr int NKP = (3 * Order<N>::Nfp + 3) / 4;   // k-steps of the face GEMM
+};
+
+template <int N, int EQN, int FLUX, typename T>
+__global__ void __launch_bounds__(64, (EQN == APL_EQN_PHMAXWELL8 ? 1 : 2)) rhs_linear_kernel(RhsArgs a) {
+  using O = Order<N>;
+  using L = LinLds<N, EQN>;
+  constexpr int M = Vars<EQN>::M;
+  constexpr int Np = O::Np, Nfp = O::Nfp;
+  constexpr int NKS1 = O::NKS1, NRT = O::NRT, NKP = LinOrder<N>::NKP;
+  constexpr int ES = Np * M;
+  typedef typename Acc<T>::type V4;
+
+  extern __shared__ double lds[];
+  double *XT = lds;
+  double *PT = lds + L::XT;
+  double *QT = PT + L::PT;
+  int *FM = (int *)(QT + L::PT);
+
+  const int lane = threadIdx.x;
+  const int el = lane & 15, lg = lane >> 4;
+  const long e0 = (long)blockIdx.x * 16;
+  const int cnt = (int)min((long)16, (long)a.K - e0);
+
+  const long ge = e0 + (el < cnt ? el : 0);
+  const double *gk = a.geom + ge * G_WORDS;
+  const double jrx = gk[G_JRX], jry = gk[G_JRY], jsx = gk[G_JSX], jsy = gk[G_JSY];
+  const double Jel = gk[G_J];
+  const double nx0 = gk[G_NX], nx1 = gk[G_NX + 1], nx2 = gk[G_NX + 2];
+  const double ny0 = gk[G_NY], ny1 = gk[G_NY + 1], ny2 = gk[G_NY + 2];
+  const double sj0 = gk[G_SJ], sj1 = gk[G_SJ + 1], sj2 = gk[G_SJ + 2];
+  const int code0 = a.fcode[ge * 3], code1 = a.fcode[ge * 3 + 1], code2 = a.fcode[ge * 3 + 2];
+
+  for (int i = lane; i < 3 * Nfp; i += 64) FM[i] = a.fmask[i];
+  stage_transposed<ES>(XT, a.q + e0 * ES, cnt, lane);
+  for (int i = 16 * ES + lane; i < L::XT; i += 64) XT[i] = 0.0;
+  __syncthreads();
+
+  // own and neighbour face nodes, in the own face's node order
+  constexpr int NGI = (16 * 3 * Nfp + 63) / 64;
+#pragma unroll
+  for (int j = 0; j < NGI; ++j) {
+    const int it = lane + 64 * j;
+    if (it < 16 * 3 * Nfp) {
+      const int e = it & 15, k = it >> 4, f = k / Nfp, i = k - f * Nfp;
+      const long gee = e0 + (e < cnt ? e : 0);
+      const int code = a.fcode[gee * 3 + f];
+      double v8[M];
+      if (code >= 0) {
+        const long nb = code >> 2;
+        const int nf = code & 3;
+        const double *src = a.q + (nb * Np + FM[nf * Nfp + (Nfp - 1 - i)]) * M;
+#pragma unroll
+        for (int v = 0; v < M; ++v) v8[v] = src[v];
+      } else {
+#pragma unroll
+        for (int v = 0; v < M; ++v) v8[v] = 0.0;
+      }
+#pragma unroll
+      for (int v = 0; v < M; ++v) {
+        PT[(k * M + v) * 16 + e] = v8[v];
+        QT[(k * M + v) * 16 + e] = XT[(FM[f * Nfp + i] * M + v) * 16 + e];
+      }
     }
   }
-  // wave-wide max speed and flags
+  __syncthreads();
+
+  unsigned ef = 0;
+  const T *A1 = (const T *)a.A1;   // [2][NRT][NKS1][64]: Dt_r, Dt_s
+  const T *A3 = (const T *)a.A3;   // [NRT][NKP][64]: -Lt
+
+  // volume at the nodes
+  T b1[NKS1][M];
+#pragma unroll
+  for (int ks = 0; ks < NKS1; ++ks)
+#pragma unroll
+    for (int v = 0; v < M; ++v) b1[ks][v] = T(XT[((4 * ks + lg) * M + v) * 16 + el]);
+
+  V4 acc[NRT][M];
+#pragma unroll
+  for (int rt = 0; rt < NRT; ++rt) {
+    T ar[NKS1], as[NKS1];
+#pragma unroll
+    for (int ks = 0; ks < NKS1; ++ks) {
+      ar[ks] = A1[((0 * NRT + rt) * NKS1 + ks) * 64 + lane];
+      as[ks] = A1[((1 * NRT + rt) * NKS1 + ks) * 64 + lane];
+    }
+    V4 pr[M], ps[M];
+#pragma unroll
+    for (int v = 0; v < M; ++v) pr[v] = ps[v] = V4{0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < NKS1; ++ks)
+#pragma unroll
+      for (int v = 0; v < M; ++v) {
+        pr[v] = mma(ar[ks], b1[ks][v], pr[v]);
+        ps[v] = mma(as[ks], b1[ks][v], ps[v]);
+      }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      T p1[M], p2[M], F1[M], G1[M], F2[M], G2[M];
+#pragma unroll
+      for (int v = 0; v < M; ++v) {
+        p1[v] = pr[v][r];
+        p2[v] = ps[v][r];
+      }
+      phys_flux<EQN, T>(a.ph, p1, F1, G1);
+      phys_flux<EQN, T>(a.ph, p2, F2, G2);
+#pragma unroll
+      for (int v = 0; v < M; ++v)
+        acc[rt][v][r] = T(jrx * double(F1[v]) + jry * double(G1[v]) + jsx * double(F2[v]) +
+                          jsy * double(G2[v]));
+    }
+  }
+
+  // surface at the face nodes kf = 4 ks + lg
+  double lmax = 0.0;
+#pragma unroll 1
+  for (int ks = 0; ks < NKP; ++ks) {
+    const int kf = 4 * ks + lg;
+    T a3[NRT];
+#pragma unroll
+    for (int rt = 0; rt < NRT; ++rt) a3[rt] = A3[(rt * NKP + ks) * 64 + lane];
+    T fsT[M];
+    if (kf < 3 * Nfp) {
+      const int f = kf / Nfp;
+      const int code = f == 0 ? code0 : (f == 1 ? code1 : code2);
+      const double nx = f == 0 ? nx0 : (f == 1 ? nx1 : nx2);
+      const double ny = f == 0 ? ny0 : (f == 1 ? ny1 : ny2);
+      const double sj = f == 0 ? sj0 : (f == 1 ? sj1 : sj2);
+      double qM[M], qP[M], fs[M];
+#pragma unroll
+      for (int v = 0; v < M; ++v) {
+        qM[v] = QT[(kf * M + v) * 16 + el];
+        qP[v] = PT[(kf * M + v) * 16 + el];
+      }
+      if (code < 0) boundary_state<EQN>(a.bc, -code - 1, 0.0, 0.0, 0.0, nx, ny, qM, qP);
+      double lam;
+      ef |= num_flux<EQN, FLUX>(a.ph, nx, ny, qM, qP, fs, &lam);
+      lmax = lam > lmax ? lam : lmax;
+#pragma unroll
+      for (int v = 0; v < M; ++v) fsT[v] = T(fs[v] * sj);
+    } else {
+#pragma unroll
+      for (int v = 0; v < M; ++v) fsT[v] = T(0);
+    }
+#pragma unroll
+    for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+      for (int v = 0; v < M; ++v) acc[rt][v] = mma(a3[rt], fsT[v], acc[rt][v]);
+  }
+
+  // epilogue: / J, stage through LDS as [e][n][v], coalesced write
+  __syncthreads();
+  double *OUT = lds;
+#pragma unroll
+  for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n = 16 * rt + acc_row<T>(lg, r);
+      if (n < Np) {
+#pragma unroll
+        for (int v = 0; v < M; ++v) OUT[el * ES + n * M + v] = double(acc[rt][v][r]) / Jel;
+      }
+    }
+  __syncthreads();
+  ef |= write_batch<ES>(a, OUT, e0, cnt, lane);
   for (int off = 32; off > 0; off >>= 1) {
     const double o = __shfl_xor(lmax, off);
     lmax = o > lmax ? o : lmax;
@@ -506,8 +853,10 @@ __global__ void __launch_bounds__(64) rhs_kernel(RhsArgs a) {
 // Advance the device clock by dt at the end of a captured step.
 int launch_advance_time(hipStream_t s, double *t, double dt);
 
-// Launch entry of one order (defined in apl_kernels_N<N>.hip).
+// Launch entry of one order (defined in apl_kernels_N<N>.hip); `linear`
+// selects the nodal kernel (Maxwell only).
 template <int N>
-int launch_rhs(int eqn, int flux, int prec, hipStream_t s, const RhsArgs &a, int nblocks);
+int launch_rhs(int eqn, int flux, int prec, int linear, hipStream_t s, const RhsArgs &a,
+               int nblocks);
 
 }  // namespace apl

@@ -18,7 +18,8 @@
 
 namespace apl {
 template <int N>
-int launch_rhs(int eqn, int flux, int prec, hipStream_t s, const RhsArgs &a, int nblocks);
+int launch_rhs(int eqn, int flux, int prec, int linear, hipStream_t s, const RhsArgs &a,
+               int nblocks);
 }  // namespace apl
 
 using namespace apl;
@@ -28,6 +29,7 @@ static thread_local std::string g_create_error;
 struct apl_handle_s {
   apl_params params;
   int K = 0, M = 0, Nv = 0;
+  bool linear = false;   // nodal linear-flux kernel (Maxwell, APL_ALGO_AUTO)
   Operators ops;
   Geometry geo;
   std::vector<int> etetf, fcode, vmapP;
@@ -98,16 +100,17 @@ static int nvars(int eqn) {
 static int dispatch_rhs(apl_handle *h, const RhsArgs &a) {
   const int nb = (h->K + 15) / 16;
   const apl_params &p = h->params;
+  const int lin = h->linear ? 1 : 0;
   int rc;
   switch (p.order) {
-    case 1: rc = launch_rhs<1>(p.eqn, p.flux, p.precision, h->stream, a, nb); break;
-    case 2: rc = launch_rhs<2>(p.eqn, p.flux, p.precision, h->stream, a, nb); break;
-    case 3: rc = launch_rhs<3>(p.eqn, p.flux, p.precision, h->stream, a, nb); break;
-    case 4: rc = launch_rhs<4>(p.eqn, p.flux, p.precision, h->stream, a, nb); break;
-    case 5: rc = launch_rhs<5>(p.eqn, p.flux, p.precision, h->stream, a, nb); break;
-    case 6: rc = launch_rhs<6>(p.eqn, p.flux, p.precision, h->stream, a, nb); break;
-    case 7: rc = launch_rhs<7>(p.eqn, p.flux, p.precision, h->stream, a, nb); break;
-    case 8: rc = launch_rhs<8>(p.eqn, p.flux, p.precision, h->stream, a, nb); break;
+    case 1: rc = launch_rhs<1>(p.eqn, p.flux, p.precision, lin, h->stream, a, nb); break;
+    case 2: rc = launch_rhs<2>(p.eqn, p.flux, p.precision, lin, h->stream, a, nb); break;
+    case 3: rc = launch_rhs<3>(p.eqn, p.flux, p.precision, lin, h->stream, a, nb); break;
+    case 4: rc = launch_rhs<4>(p.eqn, p.flux, p.precision, lin, h->stream, a, nb); break;
+    case 5: rc = launch_rhs<5>(p.eqn, p.flux, p.precision, lin, h->stream, a, nb); break;
+    case 6: rc = launch_rhs<6>(p.eqn, p.flux, p.precision, lin, h->stream, a, nb); break;
+    case 7: rc = launch_rhs<7>(p.eqn, p.flux, p.precision, lin, h->stream, a, nb); break;
+    case 8: rc = launch_rhs<8>(p.eqn, p.flux, p.precision, lin, h->stream, a, nb); break;
     default: rc = APL_ERR_ORDER;
   }
   if (rc) return fail(h, rc, "RHS kernel launch failed");
@@ -204,10 +207,76 @@ static void make_fragments(const Operators &o, std::vector<T> &A1, std::vector<T
       }
 }
 
+// Fragments of the nodal linear-flux kernel: A1 = [Dt_r; Dt_s] with
+// Dt_h = A_h V_c, A3 = -Lt with Lt = L blkdiag(I1D_f) (see apl_device.h).
+template <typename T>
+static void make_linear_fragments(const Operators &o, std::vector<T> &A1, std::vector<T> &A3) {
+  const int Np = o.Np, Nc = o.Nc, Ng = o.Ng, Nfp = o.Nfp;
+  const int NKS1 = (Np + 3) / 4, NRT = (Np + 15) / 16, NKP = (3 * Nfp + 3) / 4;
+  std::vector<double> D[2] = {std::vector<double>((size_t)Np * Np, 0.0),
+                              std::vector<double>((size_t)Np * Np, 0.0)};
+  std::vector<double> Lt((size_t)Np * 3 * Nfp, 0.0);
+  for (int n = 0; n < Np; ++n) {
+    // rows of A_r, A_s (V V^T Dr_c^T diag(w)) and L (V V^T I_f^T diag(gw))
+    std::vector<double> ar(Nc), as(Nc), lr(3 * Ng);
+    for (int c = 0; c < Nc; ++c) {
+      double sr = 0.0, ss = 0.0;
+      for (int j = 0; j < Np; ++j) {
+        sr += o.VVT[(size_t)n * Np + j] * o.Drc[(size_t)c * Np + j];
+        ss += o.VVT[(size_t)n * Np + j] * o.Dsc[(size_t)c * Np + j];
+      }
+      ar[c] = sr * o.cw[c];
+      as[c] = ss * o.cw[c];
+    }
+    for (int k = 0; k < 3 * Ng; ++k) {
+      double sl = 0.0;
+      for (int j = 0; j < Np; ++j) sl += o.VVT[(size_t)n * Np + j] * o.If[(size_t)k * Np + j];
+      lr[k] = sl * o.gw[k % Ng];
+    }
+    for (int j = 0; j < Np; ++j) {
+      double sr = 0.0, ss = 0.0;
+      for (int c = 0; c < Nc; ++c) {
+        sr += ar[c] * o.Vc[(size_t)c * Np + j];
+        ss += as[c] * o.Vc[(size_t)c * Np + j];
+      }
+      D[0][(size_t)n * Np + j] = sr;
+      D[1][(size_t)n * Np + j] = ss;
+    }
+    for (int f = 0; f < 3; ++f)
+      for (int i = 0; i < Nfp; ++i) {
+        double sl = 0.0;
+        for (int g = 0; g < Ng; ++g)
+          sl += lr[f * Ng + g] * o.If[((size_t)f * Ng + g) * Np + o.fmask[f * Nfp + i]];
+        Lt[(size_t)n * 3 * Nfp + f * Nfp + i] = sl;
+      }
+  }
+  A1.assign((size_t)2 * NRT * NKS1 * 64, T(0));
+  for (int hh = 0; hh < 2; ++hh)
+    for (int rt = 0; rt < NRT; ++rt)
+      for (int ks = 0; ks < NKS1; ++ks)
+        for (int l = 0; l < 64; ++l) {
+          const int n = 16 * rt + (l & 15), j = 4 * ks + (l >> 4);
+          if (n < Np && j < Np)
+            A1[(((size_t)hh * NRT + rt) * NKS1 + ks) * 64 + l] = T(D[hh][(size_t)n * Np + j]);
+        }
+  A3.assign((size_t)NRT * NKP * 64, T(0));
+  for (int rt = 0; rt < NRT; ++rt)
+    for (int ks = 0; ks < NKP; ++ks)
+      for (int l = 0; l < 64; ++l) {
+        const int n = 16 * rt + (l & 15), k = 4 * ks + (l >> 4);
+        if (n < Np && k < 3 * Nfp) A3[((size_t)rt * NKP + ks) * 64 + l] = T(-Lt[(size_t)n * 3 * Nfp + k]);
+      }
+}
+
 template <typename T>
 static int upload_fragments(apl_handle *h) {
   std::vector<T> A1, A2, A3;
-  make_fragments<T>(h->ops, A1, A2, A3);
+  if (h->linear) {
+    make_linear_fragments<T>(h->ops, A1, A3);
+    A2.assign(1, T(0));
+  } else {
+    make_fragments<T>(h->ops, A1, A2, A3);
+  }
   int rc;
   if ((rc = dev_upload<T>(h, (T **)&h->d_A1, A1.data(), A1.size()))) return rc;
   if ((rc = dev_upload<T>(h, (T **)&h->d_A2, A2.data(), A2.size()))) return rc;
@@ -302,6 +371,8 @@ int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out)
   if (params->precision != APL_PREC_FP64 && params->precision != APL_PREC_FP32_MFMA &&
       params->precision != APL_PREC_BF16X3)
     return fail(nullptr, APL_ERR_ARG, "unknown precision");
+  if (params->algorithm != APL_ALGO_AUTO && params->algorithm != APL_ALGO_CUBATURE)
+    return fail(nullptr, APL_ERR_ARG, "unknown algorithm");
   if (params->eqn == APL_EQN_MAXWELL_TM3 && (params->chi != 0.0 || params->gamma_ph != 0.0))
     return fail(nullptr, APL_ERR_ARG, "MAXWELL_TM3 requires chi == gamma == 0");
   if (params->n_bc < 0 || params->n_bc > APL_MAX_BC) return fail(nullptr, APL_ERR_ARG, "n_bc");
@@ -313,6 +384,8 @@ int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out)
 
   apl_handle *h = new apl_handle();
   h->params = *params;
+  h->linear = params->algorithm == APL_ALGO_AUTO &&
+              (params->eqn == APL_EQN_MAXWELL_TM3 || params->eqn == APL_EQN_PHMAXWELL8);
   h->K = mesh->K;
   h->M = M;
   h->Nv = mesh->Nv;

@@ -10,7 +10,20 @@
 namespace apl {
 
 template <int N, int EQN, int FLUX, typename T>
-static int launch_typed(hipStream_t s, const RhsArgs &a, int nblocks) {
+static int launch_typed(int linear, hipStream_t s, const RhsArgs &a, int nblocks) {
+  if constexpr (EQN == APL_EQN_PHMAXWELL8 || EQN == APL_EQN_MAXWELL_TM3) {
+    if (linear) {
+      const size_t lds = sizeof(double) * LinLds<N, EQN>::doubles;
+      auto kern = rhs_linear_kernel<N, EQN, FLUX, T>;
+      if (lds > 65536 && hipFuncSetAttribute((const void *)kern,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)lds) != hipSuccess)
+        return APL_ERR_UNSUPPORTED;
+      hipLaunchKernelGGL(kern, dim3(nblocks), dim3(64), lds, s, a);
+      return hipGetLastError() == hipSuccess ? APL_OK : APL_ERR_HIP;
+    }
+  }
+  if (linear) return APL_ERR_ARG;
   const size_t lds = sizeof(double) * Lds<N, EQN>::doubles;
   auto kern = rhs_kernel<N, EQN, FLUX, T>;
   if (lds > 65536) {
@@ -23,30 +36,33 @@ static int launch_typed(hipStream_t s, const RhsArgs &a, int nblocks) {
 }
 
 template <int N, int EQN, int FLUX>
-static int launch_prec(int prec, hipStream_t s, const RhsArgs &a, int nblocks) {
+static int launch_prec(int prec, int linear, hipStream_t s, const RhsArgs &a, int nblocks) {
   switch (prec) {
-    case APL_PREC_FP64: return launch_typed<N, EQN, FLUX, double>(s, a, nblocks);
-    case APL_PREC_FP32_MFMA: return launch_typed<N, EQN, FLUX, float>(s, a, nblocks);
+    case APL_PREC_FP64: return launch_typed<N, EQN, FLUX, double>(linear, s, a, nblocks);
+    case APL_PREC_FP32_MFMA: return launch_typed<N, EQN, FLUX, float>(linear, s, a, nblocks);
     default: return APL_ERR_UNSUPPORTED;
   }
 }
 
 template <int N>
-int launch_rhs(int eqn, int flux, int prec, hipStream_t s, const RhsArgs &a, int nblocks) {
+int launch_rhs(int eqn, int flux, int prec, int linear, hipStream_t s, const RhsArgs &a,
+               int nblocks) {
   switch (eqn) {
-    case APL_EQN_EULER5: return launch_prec<N, APL_EQN_EULER5, APL_FLUX_LF>(prec, s, a, nblocks);
-    case APL_EQN_EULER4: return launch_prec<N, APL_EQN_EULER4, APL_FLUX_LF>(prec, s, a, nblocks);
+    case APL_EQN_EULER5:
+      return launch_prec<N, APL_EQN_EULER5, APL_FLUX_LF>(prec, linear, s, a, nblocks);
+    case APL_EQN_EULER4:
+      return launch_prec<N, APL_EQN_EULER4, APL_FLUX_LF>(prec, linear, s, a, nblocks);
     case APL_EQN_PHMAXWELL8:
-      return launch_prec<N, APL_EQN_PHMAXWELL8, APL_FLUX_LF>(prec, s, a, nblocks);
+      return launch_prec<N, APL_EQN_PHMAXWELL8, APL_FLUX_LF>(prec, linear, s, a, nblocks);
     case APL_EQN_MAXWELL_TM3:
       if (flux == APL_FLUX_UPWIND)
-        return launch_prec<N, APL_EQN_MAXWELL_TM3, APL_FLUX_UPWIND>(prec, s, a, nblocks);
-      return launch_prec<N, APL_EQN_MAXWELL_TM3, APL_FLUX_LF>(prec, s, a, nblocks);
+        return launch_prec<N, APL_EQN_MAXWELL_TM3, APL_FLUX_UPWIND>(prec, linear, s, a, nblocks);
+      return launch_prec<N, APL_EQN_MAXWELL_TM3, APL_FLUX_LF>(prec, linear, s, a, nblocks);
     default: return APL_ERR_ARG;
   }
 }
 
-template int launch_rhs<APL_ORDER>(int, int, int, hipStream_t, const RhsArgs &, int);
+template int launch_rhs<APL_ORDER>(int, int, int, int, hipStream_t, const RhsArgs &, int);
 
 #if APL_ORDER == 1
 __global__ void advance_time_kernel(double *t, double dt) {

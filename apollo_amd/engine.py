@@ -21,6 +21,7 @@ APL_OK = 0
 EQN_EULER5, EQN_EULER4, EQN_PHMAXWELL8, EQN_MAXWELL_TM3 = 0, 1, 2, 3
 FLUX_LF, FLUX_UPWIND = 0, 1
 PREC_FP64, PREC_FP32_MFMA, PREC_BF16X3 = 0, 1, 2
+ALGO_AUTO, ALGO_CUBATURE = 0, 1
 BC_NONE, BC_ISENTROPIC_VORTEX, BC_MAXWELL_TM, BC_PEC, BC_COPY = 0, 1, 2, 3, 4
 HOST, DEVICE = 0, 1
 MAX_BC = 8
@@ -47,7 +48,8 @@ class _Mesh(ctypes.Structure):
 
 class _Params(ctypes.Structure):
     _fields_ = [("order", ctypes.c_int), ("eqn", ctypes.c_int), ("flux", ctypes.c_int),
-                ("precision", ctypes.c_int), ("gas_gamma", ctypes.c_double),
+                ("precision", ctypes.c_int), ("algorithm", ctypes.c_int),
+                ("gas_gamma", ctypes.c_double),
                 ("min_density", ctypes.c_double), ("min_pressure", ctypes.c_double),
                 ("c0", ctypes.c_double), ("chi", ctypes.c_double), ("gamma_ph", ctypes.c_double),
                 ("n_bc", ctypes.c_int), ("bc", _BC * MAX_BC)]
@@ -202,8 +204,10 @@ class Params:
     """apl_params with keyword defaults."""
 
     def __init__(self, order, eqn, flux=FLUX_LF, precision=PREC_FP64, gas_gamma=1.4,
-                 min_density=0.0, min_pressure=0.0, c0=1.0, chi=0.0, gamma_ph=0.0, bcs=()):
+                 min_density=0.0, min_pressure=0.0, c0=1.0, chi=0.0, gamma_ph=0.0, bcs=(),
+                 algorithm=ALGO_AUTO):
         self.order, self.eqn, self.flux, self.precision = order, eqn, flux, precision
+        self.algorithm = algorithm
         self.gas_gamma, self.min_density, self.min_pressure = gas_gamma, min_density, min_pressure
         self.c0, self.chi, self.gamma_ph = c0, chi, gamma_ph
         self.bcs = list(bcs)
@@ -211,6 +215,7 @@ class Params:
     def c_struct(self):
         p = _Params()
         p.order, p.eqn, p.flux, p.precision = self.order, self.eqn, self.flux, self.precision
+        p.algorithm = self.algorithm
         p.gas_gamma, p.min_density, p.min_pressure = self.gas_gamma, self.min_density, self.min_pressure
         p.c0, p.chi, p.gamma_ph = self.c0, self.chi, self.gamma_ph
         p.n_bc = len(self.bcs)

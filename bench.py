@@ -82,16 +82,29 @@ def flop_alg(K, N, Np, Nc, Ng, m):
     return K * 2 * m * (3 * Nc * Np + 6 * Ng * Np)
 
 
+def flop_exec(variant, K, N, Np, Nc, Ng, m):
+    """Contraction flops the engine's kernel actually performs per RHS (no
+    padding): the cubature form does V_c (Nc Np), A_r|A_s (2 Nc Np), the face
+    traces (2 x 3 Ng (N+1)) and L (3 Ng Np) per variable; the nodal form of a
+    linear flux does Dt_r|Dt_s (2 Np^2) and Lt (3 (N+1) Np)."""
+    Nfp = N + 1
+    if variant in ("fp64", "fp32") and m == 3:
+        per = 2 * Np * Np + 3 * Nfp * Np
+    else:
+        per = 3 * Nc * Np + 6 * Ng * Nfp + 3 * Ng * Np
+    return K * 2 * m * per
+
+
 def bytes_alg_stage(K, Np, m):
     """HBM bytes of one fused RHS + LSERK4 stage: read q and res, write q and
     res (FP64), plus 136 B of geometry and connectivity per element."""
     return K * (4 * Np * m * 8 + 136)
 
 
-def run_gpu(w, precision, steps, warmup, torch):
+def run_gpu(w, precision, steps, warmup, torch, algorithm=0):
     from apollo_amd import engine as E
     p = dict(w["params"])
-    eng = E.Engine(w["mesh"], E.Params(precision=precision, **p))
+    eng = E.Engine(w["mesh"], E.Params(precision=precision, algorithm=algorithm, **p))
     stream = torch.cuda.Stream()
     eng_stream = stream.cuda_stream
     from apollo_amd.engine import lib
@@ -158,7 +171,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--only", default="", help="fp64 or fp32: time one precision only")
+    ap.add_argument("--only", default="", help="time one variant only (fp64, fp32, fp64_cub, fp32_cub)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the multi-GPU design is "
@@ -169,11 +182,17 @@ def main():
     assert torch.cuda.is_available()
     w = workload(args.workload)
     res = {}
-    for name, prec in (("fp64", E.PREC_FP64), ("fp32", E.PREC_FP32_MFMA)):
+    variants = [("fp64", E.PREC_FP64, E.ALGO_AUTO), ("fp32", E.PREC_FP32_MFMA, E.ALGO_AUTO)]
+    if w["m"] == 3:
+        # Maxwell: the default is the nodal form of the linear flux; also time
+        # Apollo's over-integrated cubature form
+        variants += [("fp64_cub", E.PREC_FP64, E.ALGO_CUBATURE),
+                     ("fp32_cub", E.PREC_FP32_MFMA, E.ALGO_CUBATURE)]
+    for name, prec, algo in variants:
         if args.only and args.only != name:
             continue
         print("bench: %s %s steps=%d" % (args.workload, name, args.steps), file=sys.stderr, flush=True)
-        res[name] = run_gpu(w, prec, args.steps, args.warmup, torch)
+        res[name] = run_gpu(w, prec, args.steps, args.warmup, torch, algo)
     info = next(iter(res.values()))["info"]
     K, N, Np, Nc, Ng, m = (info[k] for k in ("K", "N", "Np", "Nc", "Ng", "m"))
     rhs_evals = 5 * args.steps
@@ -190,6 +209,7 @@ def main():
         out[name] = dict(dof_updates_per_s=dofs, ms_per_step=ms / args.steps,
                          kernel_us=kern_s * 1e6, l2_vs_analytic=err_exact, T=T,
                          tflops_alg=flop_alg(K, N, Np, Nc, Ng, m) / kern_s / 1e12,
+                         tflops_exec=flop_exec(name, K, N, Np, Nc, Ng, m) / kern_s / 1e12,
                          gbs_alg=bytes_alg_stage(K, Np, m) / kern_s / 1e9)
     head = "fp32" if "fp32" in out else "fp64"
     l2_32_64 = None

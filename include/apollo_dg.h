@@ -69,6 +69,16 @@ enum apl_precision {
                               FP32 accumulation; FP64 as above               */
 };
 
+enum apl_algorithm {
+  APL_ALGO_AUTO = 0,       /* cubature form for nonlinear fluxes (Euler); for
+                              linear fluxes (Maxwell) the algebraically
+                              identical nodal form V V^T Dr_c^T W V_c Q ... on
+                              straight-sided elements (no reference
+                              counterpart; same result to rounding)          */
+  APL_ALGO_CUBATURE = 1    /* always Apollo's over-integrated cubature form
+                              (wxcubature2d.cc:288-382)                      */
+};
+
 enum apl_bc_kind {
   APL_BC_NONE = 0,
   APL_BC_ISENTROPIC_VORTEX = 1, /* exact vortex state, wxisentropicvortexbc.cc:20-43
@@ -109,6 +119,7 @@ typedef struct {
   int eqn;               /* apl_eqn                                            */
   int flux;              /* apl_flux                                           */
   int precision;         /* apl_precision                                      */
+  int algorithm;         /* apl_algorithm                                      */
   double gas_gamma;      /* Euler `gas_gamma`                                  */
   double min_density;    /* Euler `minDensity`  (0: rho<=0 is an error)        */
   double min_pressure;   /* Euler `minPressure` (0: p<=0 is an error)          */

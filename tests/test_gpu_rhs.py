@@ -17,12 +17,14 @@ pytestmark = pytest.mark.gpu
 FP64_TOL = 1e-12
 
 
-def _rhs(case, q, t=0.0, precision=None):
+def _rhs(case, q, t=0.0, precision=None, algorithm=None):
     params = case.params
-    if precision is not None:
-        params = E.Params(params.order, params.eqn, flux=params.flux, precision=precision,
+    if precision is not None or algorithm is not None:
+        params = E.Params(params.order, params.eqn, flux=params.flux,
+                          precision=params.precision if precision is None else precision,
                           gas_gamma=params.gas_gamma, c0=params.c0, chi=params.chi,
-                          gamma_ph=params.gamma_ph, bcs=params.bcs)
+                          gamma_ph=params.gamma_ph, bcs=params.bcs,
+                          algorithm=params.algorithm if algorithm is None else algorithm)
     eng = E.Engine(case.mesh, params)
     out = np.zeros_like(q)
     eng.rhs(t, q, out)
@@ -50,6 +52,21 @@ def test_maxwell_rhs_fp64(bc, m):
     case = maxwell_case(n=10, N=5, m=m, bc=bc)
     q = case.state()
     fp64_parity(case, _rhs(case, q), q)
+
+
+@pytest.mark.parametrize("N", [2, 5, 8])
+@pytest.mark.parametrize("m", [3, 8])
+def test_maxwell_cubature_and_nodal_forms_agree(N, m):
+    """Maxwell runs the nodal linear-flux kernel by default; Apollo's cubature
+    form (APL_ALGO_CUBATURE) must give the same RHS, and both match the oracle."""
+    for bc, flux in ((E.BC_MAXWELL_TM, E.FLUX_LF), (E.BC_PEC, E.FLUX_UPWIND if m == 3 else E.FLUX_LF)):
+        case = maxwell_case(n=6, N=N, m=m, bc=bc, flux=flux)
+        q = case.state()
+        nodal = _rhs(case, q)
+        cub = _rhs(case, q, algorithm=E.ALGO_CUBATURE)
+        fp64_parity(case, nodal, q)
+        fp64_parity(case, cub, q)
+        assert rel_l2(nodal, cub) <= 1e-12
 
 
 def test_phmaxwell_chi_gamma_fp64():
