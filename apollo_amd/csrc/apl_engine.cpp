@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/apollo_dg.h"
+#include "apl_adjoint.h"
 #include "apl_device.h"
 #include "apl_host.h"
 
@@ -20,6 +21,10 @@ namespace apl {
 template <int N>
 int launch_rhs(int eqn, int flux, int prec, int linear, hipStream_t s, const RhsArgs &a,
                int nblocks);
+template <int N>
+int launch_adjoint(int eqn, int flux, hipStream_t s, const AdjArgs &a, int nblocks);
+template <int N>
+int launch_objective(int m, hipStream_t s, const ObjArgs &a, int nblocks);
 }  // namespace apl
 
 using namespace apl;
@@ -50,10 +55,16 @@ struct apl_handle_s {
   hipGraph_t step_graph[2] = {nullptr, nullptr};
   const double *graph_qa = nullptr;
   double graph_dt = 0.0;
-  // adjoint work buffers
-  double *d_adj[8] = {nullptr};
+  // adjoint: FP64 operator fragments, work buffers, checkpoints
+  double *d_adjops[6] = {nullptr};   // Avc, Aart, Aast, Avct, Alt, Aift
+  double *d_Vc = nullptr, *d_cub = nullptr;
+  double *d_sbar = nullptr, *d_qbar = nullptr, *d_rbar = nullptr;
+  double *d_stg = nullptr;           // 6 stage buffers
+  double *d_seg = nullptr;           // states of one checkpoint segment
+  long long seg_slots = 0;
   double *d_ckpt = nullptr;
   long long ckpt_slots = 0;
+  double *d_opart = nullptr;
 };
 
 // ---------------------------------------------------------------------------
@@ -299,10 +310,11 @@ static void free_handle(apl_handle *h) {
   free_graphs(h);
   void *bufs[] = {h->d_geom, h->d_vert, h->d_I1D, h->d_fcode, h->d_fmask, h->d_A1, h->d_A2,
                   h->d_A3, h->d_err, h->d_speed, h->d_t, h->d_qa, h->d_qb, h->d_res, h->d_out,
-                  h->d_ckpt};
+                  h->d_ckpt, h->d_Vc, h->d_cub, h->d_sbar, h->d_qbar, h->d_rbar, h->d_stg,
+                  h->d_seg, h->d_opart};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
-  for (double *b : h->d_adj)
+  for (double *b : h->d_adjops)
     if (b) (void)hipFree(b);
   if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -749,18 +761,303 @@ int apl_lserk4(apl_handle *h, double t0, double dt, int nsteps, double *q, int w
   return run_lserk4(h, t0, dt, nsteps, q, h->d_qb);
 }
 
-int apl_adjoint_rhs(apl_handle *h, double, const double *, const double *, double *, int) {
-  return h ? fail(h, APL_ERR_UNSUPPORTED, "adjoint not built yet") : APL_ERR_ARG;
+// ---------------------------------------------------------------------------
+// discrete adjoint
+// ---------------------------------------------------------------------------
+
+// FP64 fragments of the transposed operators (see apl_adjoint.h).
+static int ensure_adjoint_ops(apl_handle *h) {
+  if (h->d_adjops[0]) return APL_OK;
+  const Operators &o = h->ops;
+  const int Np = o.Np, Nc = o.Nc, Ng = o.Ng;
+  const int NKS1 = (Np + 3) / 4, NCH = (Nc + 15) / 16, NRT = (Np + 15) / 16, NKF = (3 * Ng + 3) / 4;
+  const int NFT = (3 * Ng + 15) / 16;
+  std::vector<double> Ar((size_t)Np * Nc), As((size_t)Np * Nc), L((size_t)Np * 3 * Ng);
+  for (int n = 0; n < Np; ++n) {
+    for (int c = 0; c < Nc; ++c) {
+      double sr = 0.0, ss = 0.0;
+      for (int j = 0; j < Np; ++j) {
+        sr += o.VVT[(size_t)n * Np + j] * o.Drc[(size_t)c * Np + j];
+        ss += o.VVT[(size_t)n * Np + j] * o.Dsc[(size_t)c * Np + j];
+      }
+      Ar[(size_t)n * Nc + c] = sr * o.cw[c];
+      As[(size_t)n * Nc + c] = ss * o.cw[c];
+    }
+    for (int k = 0; k < 3 * Ng; ++k) {
+      double sl = 0.0;
+      for (int j = 0; j < Np; ++j) sl += o.VVT[(size_t)n * Np + j] * o.If[(size_t)k * Np + j];
+      L[(size_t)n * 3 * Ng + k] = sl * o.gw[k % Ng];
+    }
+  }
+  std::vector<double> vc((size_t)NCH * NKS1 * 64, 0.0), ar(vc.size(), 0.0), as(vc.size(), 0.0);
+  for (int cc = 0; cc < NCH; ++cc)
+    for (int ks = 0; ks < NKS1; ++ks)
+      for (int l = 0; l < 64; ++l) {
+        const int c = 16 * cc + (l & 15), j = 4 * ks + (l >> 4);
+        if (c < Nc && j < Np) {
+          const size_t i = ((size_t)cc * NKS1 + ks) * 64 + l;
+          vc[i] = o.Vc[(size_t)c * Np + j];
+          ar[i] = Ar[(size_t)j * Nc + c];
+          as[i] = As[(size_t)j * Nc + c];
+        }
+      }
+  std::vector<double> vct((size_t)NRT * NCH * 64 * 4, 0.0);
+  for (int rt = 0; rt < NRT; ++rt)
+    for (int cc = 0; cc < NCH; ++cc)
+      for (int l = 0; l < 64; ++l)
+        for (int r = 0; r < 4; ++r) {
+          const int n = 16 * rt + (l & 15), c = 16 * cc + acc_row<double>(l >> 4, r);
+          if (n < Np && c < Nc) vct[(((size_t)rt * NCH + cc) * 64 + l) * 4 + r] = o.Vc[(size_t)c * Np + n];
+        }
+  std::vector<double> lt((size_t)NFT * NKS1 * 64, 0.0);
+  for (int ft = 0; ft < NFT; ++ft)
+    for (int ks = 0; ks < NKS1; ++ks)
+      for (int l = 0; l < 64; ++l) {
+        const int kf = 16 * ft + (l & 15), n = 4 * ks + (l >> 4);
+        if (kf < 3 * Ng && n < Np) lt[((size_t)ft * NKS1 + ks) * 64 + l] = -L[(size_t)n * 3 * Ng + kf];
+      }
+  std::vector<double> ift((size_t)NRT * NKF * 64, 0.0);
+  for (int rt = 0; rt < NRT; ++rt)
+    for (int ks = 0; ks < NKF; ++ks)
+      for (int l = 0; l < 64; ++l) {
+        const int n = 16 * rt + (l & 15), kf = 4 * ks + (l >> 4);
+        if (n < Np && kf < 3 * Ng) ift[((size_t)rt * NKF + ks) * 64 + l] = o.If[(size_t)kf * Np + n];
+      }
+  const std::vector<double> *src[6] = {&vc, &ar, &as, &vct, &lt, &ift};
+  int rc;
+  for (int i = 0; i < 6; ++i)
+    if ((rc = dev_upload<double>(h, &h->d_adjops[i], src[i]->data(), src[i]->size()))) return rc;
+  std::vector<double> cub((size_t)3 * Nc);
+  for (int c = 0; c < Nc; ++c) {
+    cub[3 * c] = o.cr[c];
+    cub[3 * c + 1] = o.cs[c];
+    cub[3 * c + 2] = o.cw[c];
+  }
+  if ((rc = dev_upload<double>(h, &h->d_Vc, o.Vc.data(), o.Vc.size()))) return rc;
+  if ((rc = dev_upload<double>(h, &h->d_cub, cub.data(), cub.size()))) return rc;
+  const size_t n = state_len(h);
+  if ((rc = dev_alloc(h, (void **)&h->d_sbar, sizeof(double) * (size_t)h->K * 3 * Ng * h->M))) return rc;
+  if ((rc = dev_alloc(h, (void **)&h->d_qbar, sizeof(double) * n))) return rc;
+  if ((rc = dev_alloc(h, (void **)&h->d_rbar, sizeof(double) * n))) return rc;
+  if ((rc = dev_alloc(h, (void **)&h->d_stg, sizeof(double) * n * 6))) return rc;
+  return APL_OK;
 }
 
-int apl_lserk4_adjoint(apl_handle *h, double, double, int, int, const double *, const double *,
-                       double *, double *, int) {
-  return h ? fail(h, APL_ERR_UNSUPPORTED, "adjoint not built yet") : APL_ERR_ARG;
+static AdjArgs adj_args(apl_handle *h) {
+  AdjArgs a;
+  std::memset(&a, 0, sizeof(a));
+  RhsArgs r = base_args(h);
+  a.K = h->K;
+  a.sbar = h->d_sbar;
+  a.geom = h->d_geom;
+  a.vert = h->d_vert;
+  a.fcode = h->d_fcode;
+  a.I1D = h->d_I1D;
+  a.fmask = h->d_fmask;
+  a.Avc = h->d_adjops[0];
+  a.Aart = h->d_adjops[1];
+  a.Aast = h->d_adjops[2];
+  a.Avct = h->d_adjops[3];
+  a.Alt = h->d_adjops[4];
+  a.Aift = h->d_adjops[5];
+  a.err = h->d_err;
+  a.ph = r.ph;
+  a.bc = r.bc;
+  return a;
 }
 
-int apl_objective_disc_sq(apl_handle *h, const double *, int, double, double, double, double *,
-                          double *, int) {
-  return h ? fail(h, APL_ERR_UNSUPPORTED, "adjoint not built yet") : APL_ERR_ARG;
+static int dispatch_adjoint(apl_handle *h, const AdjArgs &a) {
+  const int nb = (h->K + 15) / 16;
+  const apl_params &p = h->params;
+  int rc;
+  switch (p.order) {
+    case 1: rc = launch_adjoint<1>(p.eqn, p.flux, h->stream, a, nb); break;
+    case 2: rc = launch_adjoint<2>(p.eqn, p.flux, h->stream, a, nb); break;
+    case 3: rc = launch_adjoint<3>(p.eqn, p.flux, h->stream, a, nb); break;
+    case 4: rc = launch_adjoint<4>(p.eqn, p.flux, h->stream, a, nb); break;
+    case 5: rc = launch_adjoint<5>(p.eqn, p.flux, h->stream, a, nb); break;
+    case 6: rc = launch_adjoint<6>(p.eqn, p.flux, h->stream, a, nb); break;
+    case 7: rc = launch_adjoint<7>(p.eqn, p.flux, h->stream, a, nb); break;
+    case 8: rc = launch_adjoint<8>(p.eqn, p.flux, h->stream, a, nb); break;
+    default: rc = APL_ERR_ORDER;
+  }
+  if (rc) return fail(h, rc, "adjoint kernel launch failed");
+  return APL_OK;
+}
+
+int apl_adjoint_rhs(apl_handle *h, double t, const double *q, const double *lam, double *out,
+                    int where) {
+  if (!h || !q || !lam || !out) return APL_ERR_ARG;
+  int rc;
+  if ((rc = ensure_adjoint_ops(h)) || (rc = ensure_work(h))) return rc;
+  const size_t bytes = sizeof(double) * state_len(h);
+  const double *dq = q, *dl = lam;
+  double *dout = out;
+  if (where == APL_HOST) {
+    HIPCHK(h, hipMemcpyAsync(h->d_qa, q, bytes, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_qb, lam, bytes, hipMemcpyHostToDevice, h->stream));
+    dq = h->d_qa;
+    dl = h->d_qb;
+    dout = h->d_out;
+  }
+  HIPCHK(h, hipMemsetAsync(h->d_err, 0, sizeof(unsigned), h->stream));
+  AdjArgs a = adj_args(h);
+  a.mode = 0;
+  a.q = dq;
+  a.lam = dl;
+  a.out = dout;
+  a.t = t;
+  if ((rc = dispatch_adjoint(h, a))) return rc;
+  if (where == APL_HOST) HIPCHK(h, hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, h->stream));
+  return check_flags(h);
+}
+
+// One forward LSERK4 step from `in`; stage inputs go to stg[0..4] when
+// `stages` is set, the result to `result`.
+static int forward_step(apl_handle *h, double t, double dt, const double *in, double *result,
+                        bool stages) {
+  const size_t n = state_len(h);
+  double *stg = h->d_stg;
+  HIPCHK(h, hipMemcpyAsync(stg, in, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
+  RhsArgs a = base_args(h);
+  a.mode = 1;
+  a.res = h->d_res;
+  a.dt = dt;
+  a.t = t;
+  for (int s = 0; s < 5; ++s) {
+    a.q = stg + (stages ? s : (s & 1)) * n;
+    a.out = (s == 4) ? result : stg + (stages ? s + 1 : ((s + 1) & 1)) * n;
+    a.rk_a = rk4a[s];
+    a.rk_b = rk4b[s];
+    a.tofs = rk4c[s] * dt;
+    int rc = dispatch_rhs(h, a);
+    if (rc) return rc;
+  }
+  return APL_OK;
+}
+
+int apl_lserk4_adjoint(apl_handle *h, double t0, double dt, int nsteps, int ckpt_every,
+                       const double *q0, const double *qbarT, double *dJdq0, double *qT,
+                       int where) {
+  if (!h || !q0 || !qbarT || !dJdq0 || nsteps < 0 || ckpt_every < 1) return APL_ERR_ARG;
+  if (h->params.precision != APL_PREC_FP64)
+    return fail(h, APL_ERR_UNSUPPORTED, "the adjoint runs on FP64 handles");
+  int rc;
+  if ((rc = ensure_adjoint_ops(h)) || (rc = ensure_work(h))) return rc;
+  const size_t n = state_len(h), bytes = sizeof(double) * n;
+  const int C = ckpt_every;
+  const long long nck = nsteps / C + 1;
+  if (h->ckpt_slots < nck) {
+    if (h->d_ckpt) (void)hipFree(h->d_ckpt);
+    h->d_ckpt = nullptr;
+    h->ckpt_slots = 0;
+    if ((rc = dev_alloc(h, (void **)&h->d_ckpt, bytes * (size_t)nck))) return rc;
+    h->ckpt_slots = nck;
+  }
+  if (h->seg_slots < C) {
+    if (h->d_seg) (void)hipFree(h->d_seg);
+    h->d_seg = nullptr;
+    h->seg_slots = 0;
+    if ((rc = dev_alloc(h, (void **)&h->d_seg, bytes * (size_t)C))) return rc;
+    h->seg_slots = C;
+  }
+  const hipMemcpyKind in_kind = where == APL_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  HIPCHK(h, hipMemsetAsync(h->d_err, 0, sizeof(unsigned), h->stream));
+  // forward sweep with checkpoints
+  double *cur = h->d_ckpt;
+  HIPCHK(h, hipMemcpyAsync(cur, q0, bytes, in_kind, h->stream));
+  for (int step = 0; step < nsteps; ++step) {
+    double *dst = ((step + 1) % C == 0) ? h->d_ckpt + (size_t)((step + 1) / C) * n : h->d_qa;
+    if ((rc = forward_step(h, t0 + step * dt, dt, cur, dst, false))) return rc;
+    cur = dst;
+  }
+  if (qT) {
+    const hipMemcpyKind out_kind = where == APL_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    HIPCHK(h, hipMemcpyAsync(qT, cur, bytes, out_kind, h->stream));
+  }
+  // reverse sweep
+  HIPCHK(h, hipMemcpyAsync(h->d_qbar, qbarT, bytes, in_kind, h->stream));
+  HIPCHK(h, hipMemsetAsync(h->d_rbar, 0, bytes, h->stream));
+  AdjArgs a = adj_args(h);
+  a.mode = 1;
+  a.lam = h->d_rbar;
+  a.qbar = h->d_qbar;
+  a.dt = dt;
+  for (long long seg = (nsteps - 1) / C; seg >= 0 && nsteps > 0; --seg) {
+    const int s0 = (int)(seg * C), s1 = std::min(nsteps, s0 + C);
+    // states q_{s0} .. q_{s1-1} of this segment
+    HIPCHK(h, hipMemcpyAsync(h->d_seg, h->d_ckpt + (size_t)seg * n, bytes, hipMemcpyDeviceToDevice,
+                             h->stream));
+    for (int i = s0 + 1; i < s1; ++i)
+      if ((rc = forward_step(h, t0 + (i - 1) * dt, dt, h->d_seg + (size_t)(i - 1 - s0) * n,
+                             h->d_seg + (size_t)(i - s0) * n, false)))
+        return rc;
+    for (int i = s1 - 1; i >= s0; --i) {
+      const double t = t0 + i * dt;
+      // stage inputs of step i in stg[0..4]
+      if ((rc = forward_step(h, t, dt, h->d_seg + (size_t)(i - s0) * n, h->d_stg + 5 * n, true)))
+        return rc;
+      for (int s = 4; s >= 0; --s) {
+        a.q = h->d_stg + (size_t)s * n;
+        a.t = t + rk4c[s] * dt;
+        a.rk_a = rk4a[s];
+        a.rk_b = rk4b[s];
+        if ((rc = dispatch_adjoint(h, a))) return rc;
+      }
+    }
+  }
+  const hipMemcpyKind out_kind = where == APL_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  HIPCHK(h, hipMemcpyAsync(dJdq0, h->d_qbar, bytes, out_kind, h->stream));
+  return check_flags(h);
+}
+
+int apl_objective_disc_sq(apl_handle *h, const double *q, int var, double xc, double yc,
+                          double radius, double *J, double *dJdq, int where) {
+  if (!h || !q || !J || var < 0 || var >= h->M) return APL_ERR_ARG;
+  int rc;
+  if ((rc = ensure_adjoint_ops(h)) || (rc = ensure_work(h))) return rc;
+  const size_t bytes = sizeof(double) * state_len(h);
+  const int nb = (h->K + 63) / 64;
+  if (!h->d_opart && (rc = dev_alloc(h, (void **)&h->d_opart, sizeof(double) * nb))) return rc;
+  const double *dq = q;
+  if (where == APL_HOST) {
+    HIPCHK(h, hipMemcpyAsync(h->d_qa, q, bytes, hipMemcpyHostToDevice, h->stream));
+    dq = h->d_qa;
+  }
+  ObjArgs a;
+  a.K = h->K;
+  a.var = var;
+  a.q = dq;
+  a.grad = dJdq ? (where == APL_HOST ? h->d_out : dJdq) : nullptr;
+  a.part = h->d_opart;
+  a.geom = h->d_geom;
+  a.vert = h->d_vert;
+  a.Vc = h->d_Vc;
+  a.cub = h->d_cub;
+  a.xc = xc;
+  a.yc = yc;
+  a.R = radius;
+  switch (h->params.order) {
+    case 1: rc = launch_objective<1>(h->M, h->stream, a, nb); break;
+    case 2: rc = launch_objective<2>(h->M, h->stream, a, nb); break;
+    case 3: rc = launch_objective<3>(h->M, h->stream, a, nb); break;
+    case 4: rc = launch_objective<4>(h->M, h->stream, a, nb); break;
+    case 5: rc = launch_objective<5>(h->M, h->stream, a, nb); break;
+    case 6: rc = launch_objective<6>(h->M, h->stream, a, nb); break;
+    case 7: rc = launch_objective<7>(h->M, h->stream, a, nb); break;
+    case 8: rc = launch_objective<8>(h->M, h->stream, a, nb); break;
+    default: rc = APL_ERR_ORDER;
+  }
+  if (rc) return fail(h, rc, "objective kernel launch failed");
+  std::vector<double> part(nb);
+  HIPCHK(h, hipMemcpyAsync(part.data(), h->d_opart, sizeof(double) * nb, hipMemcpyDeviceToHost, h->stream));
+  if (dJdq && where == APL_HOST)
+    HIPCHK(h, hipMemcpyAsync(dJdq, h->d_out, bytes, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  double sum = 0.0;
+  for (double v : part) sum += v;     // fixed order: deterministic
+  *J = sum;
+  return APL_OK;
 }
 
 }  // extern "C"
