@@ -176,3 +176,33 @@ def test_rhs_error_codes():
     with pytest.raises(E.ApolloError) as ei:
         eng.rhs(0.0, q, np.zeros_like(q))
     assert ei.value.status in (8, 9, 7)
+
+
+def test_cpp_subsolver_interface(tmp_path):
+    """The Apollo-shaped C++ interface (apollo_amd/host) driven from C++:
+    setup/init/step/advance on the Maxwell TM cavity, checked against the
+    oracle and the Python binding."""
+    import os
+    import subprocess
+    from cases import Case, walled_unit_square
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "subsolver_demo")
+    assert os.path.exists(exe), "build tests/cpp first (make -C tests/cpp)"
+    out = tmp_path / "demo.bin"
+    n, N, steps = 6, 4, 5
+    r = subprocess.run([exe, str(n), str(N), str(steps), str(out)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    raw = np.fromfile(str(out), dtype=np.float64)
+    dt0, dt, length = raw[0], raw[1], int(raw[2])
+    q0, rhs, qT = raw[3:3 + length], raw[3 + length:3 + 2 * length], raw[3 + 2 * length:]
+    case = Case(walled_unit_square(n), E.Params(N, E.EQN_MAXWELL_TM3, bcs=[(E.BC_MAXWELL_TM, [])]))
+    q0 = q0.reshape(case.shape)
+    fp64_parity(case, rhs.reshape(case.shape), q0)
+    cfl = 0.5 / (2 * N + 1)
+    assert abs(dt0 - O.initial_dt(case.omesh, case.ophys, __import__("cases").to_oracle(case.params.eqn, q0), cfl)) <= 1e-14 * dt0
+    assert abs(dt - O.suggested_dt(case.omesh, cfl, 1.0)) <= 1e-14 * dt
+    eng = E.Engine(case.mesh, case.params)
+    q = q0.copy()
+    eng.lserk4(0.0, dt, steps, q)
+    assert rel_l2(qT.reshape(case.shape), q) <= 1e-14

@@ -186,3 +186,13 @@ def test_structured_mesh_rejects_bad_arguments():
         E.structured_mesh(2, 0, 1, 0, 1, 0.1, 0, True)      # periodic needs n >= 3
     with pytest.raises(E.ApolloError):
         E.structured_mesh(4, 0, 1, 0, 1, 0.6, 0, False)     # jitter must stay < 0.5
+
+
+def test_cpp_host_interface_compiles(tmp_path):
+    """apollo_amd/host/wx_nodal_dg2d_hip.hpp compiles and links against the C-ABI."""
+    import subprocess
+    exe = tmp_path / "demo"
+    src = os.path.join(ROOT, "tests", "cpp", "subsolver_demo.cpp")
+    subprocess.check_call(["g++", "-O1", "-std=c++17", "-Wall", "-Werror", "-o", str(exe), src,
+                           "-L" + os.path.join(ROOT, "apollo_amd"), "-lapollo_dg"])
+    assert exe.exists()
