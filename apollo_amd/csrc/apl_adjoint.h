@@ -231,7 +231,7 @@ __global__ void __launch_bounds__(64) adj_face_kernel(AdjArgs a) {
   double *SO = lds + AdjFaceLds<N, EQN>::LT;
 
   const int lane = threadIdx.x, el = lane & 15, lg = lane >> 4;
-  const long e0 = (long)blockIdx.x * 16;
+  const long e0 = (long)xcd_batch(blockIdx.x, gridDim.x) * 16;
   const int cnt = (int)min((long)16, (long)a.K - e0);
   double *rb = (double *)a.lam;
   for (int i = lane; i < 16 * ES; i += 64) {
@@ -312,7 +312,7 @@ __global__ void __launch_bounds__(64) adj_rhs_kernel(AdjArgs a) {
   int *FM = (int *)(I1 + L::I1D);
 
   const int lane = threadIdx.x, el = lane & 15, lg = lane >> 4;
-  const long e0 = (long)blockIdx.x * 16;
+  const long e0 = (long)xcd_batch(blockIdx.x, gridDim.x) * 16;
   const int cnt = (int)min((long)16, (long)a.K - e0);
   const long ge = e0 + (el < cnt ? el : 0);
   const double *gk = a.geom + ge * G_WORDS;

@@ -52,6 +52,14 @@ struct Order {
   static constexpr int NKF = (3 * Ng + 3) / 4;   // k-steps of the face GEMM
 };
 
+// rows of the transposed batch state staged in LDS: the K extent of the
+// first contraction, 4-row steps for FP32/FP64 MFMA and 32-row steps for
+// BF16 (rows >= Np are zero)
+template <int N, typename T>
+__host__ __device__ constexpr int xt_rows() {
+  return sizeof(T) == 1 ? (Order<N>::Np + 31) / 32 * 32 : Order<N>::NKS1 * 4;
+}
+
 template <int EQN> struct Vars;
 template <> struct Vars<APL_EQN_EULER5> { static constexpr int M = 5; };
 template <> struct Vars<APL_EQN_EULER4> { static constexpr int M = 4; };
@@ -110,14 +118,34 @@ struct RhsArgs {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+// Split-BF16x3 contractions: each operand x = hi + lo with hi = bf16(x) and
+// lo = bf16(x - hi) (round to nearest even), and a.b ~ ah.bh + ah.bl + al.bh
+// on v_mfma_f32_16x16x32_bf16 with FP32 accumulation.  Tag type of the
+// kernels' operand-type parameter; pointwise work runs in FP32.
+struct Bf16x3 {};
+
+// one lane's A fragment of a 16x16x32 step: hi and lo halves, 32 bytes
+struct __attribute__((aligned(16))) Bf8x2 {
+  bf8 hi, lo;
+};
 
 template <typename T> struct Acc;
 template <> struct Acc<double> { typedef d4 type; };
 template <> struct Acc<float> { typedef f4 type; };
+template <> struct Acc<Bf16x3> { typedef f4 type; };
+
+// scalar type of the pointwise flux work that feeds a contraction
+template <typename T> struct Scal { typedef T type; };
+template <> struct Scal<Bf16x3> { typedef float type; };
+
+template <typename T> struct IsF64 { static constexpr bool value = false; };
+template <> struct IsF64<double> { static constexpr bool value = true; };
 
 template <typename T>
 __host__ __device__ constexpr int acc_row(int g, int r) {
-  return sizeof(T) == 8 ? g + 4 * r : 4 * g + r;
+  return IsF64<T>::value ? g + 4 * r : 4 * g + r;
 }
 
 __device__ __forceinline__ d4 mma(double a, double b, d4 c) {
@@ -125,6 +153,21 @@ __device__ __forceinline__ d4 mma(double a, double b, d4 c) {
 }
 __device__ __forceinline__ f4 mma(float a, float b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// c += (ah + al)(bh + bl) without the al.bl term; the K index of a
+// 16x16x32 step is k = 8 (lane/16) + item in both operands
+__device__ __forceinline__ f4 mma3(const Bf8x2 &a, const bf8 &bh, const bf8 &bl, f4 c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.lo, bh, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, bl, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, bh, c, 0, 0, 0);
+  return c;
+}
+
+__device__ __forceinline__ void split_bf16(float x, bf8 &h, bf8 &l, int it) {
+  const __bf16 hi = (__bf16)x;
+  h[it] = hi;
+  l[it] = (__bf16)(x - (float)hi);
 }
 
 // ---------------------------------------------------------------------------
@@ -283,6 +326,18 @@ __device__ __forceinline__ void boundary_state(const BCTable &B, int idx, double
 // batch staging helpers (one wave, 16 elements of ES doubles each)
 // ---------------------------------------------------------------------------
 
+// Workgroups are dispatched to the 8 XCDs of an MI355X round-robin, and each
+// XCD has its own L2.  Batch order follows mesh order, so the face
+// neighbours of batch b sit in nearby batches: the batch a workgroup takes
+// is remapped so that each XCD walks one contiguous range of batches (a
+// bijection of [0, nb): XCD x = b % 8 takes start(x) + b / 8).
+__device__ __forceinline__ int xcd_batch(int b, int nb) {
+  constexpr int NX = 8;
+  const int q = nb / NX, r = nb - q * NX;
+  const int x = b % NX, i = b / NX;
+  return x * q + (x < r ? x : r) + i;
+}
+
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 
 // XT[w*16 + e] = q[(e0 + e)*ES + w]: all loads of a full batch are issued
@@ -386,21 +441,108 @@ __device__ __forceinline__ unsigned write_batch(const RhsArgs &a, const double *
 
 // LDS of one single-wave workgroup (doubles).  The batch state is staged
 // transposed, XT[(node*M + v)*16 + e], so that B-operand and face-node reads
-// of the 16 lanes of a lane group hit consecutive addresses; the neighbours'
-// face nodes are staged as PT[((f*Nfp + i)*M + v)*16 + e] in the own face's
-// node order (neighbour node Fmask[nf][Nfp-1-i] for own node i), and the own
-// face nodes likewise as QT (node Fmask[f][i]).  The small tables
+// of the 16 lanes of a lane group hit consecutive addresses (the own face
+// nodes are read from it through Fmask); the neighbours' face nodes are
+// staged as PT[((f*Nfp + i)*M + v)*16 + e] in the own face's node order
+// (neighbour node Fmask[nf][Nfp-1-i] for own node i).  The small tables
 // I1D[3][Ng][Nfp] and Fmask[3][Nfp] follow.
-template <int N, int EQN>
+template <int N, int EQN, typename T>
 struct Lds {
   static constexpr int M = Vars<EQN>::M;
   static constexpr int Np = Order<N>::Np, Nfp = Order<N>::Nfp, Ng = Order<N>::Ng;
-  static constexpr int XT = Order<N>::NKS1 * 4 * M * 16;   // rows >= Np are zero
+  static constexpr int XT = xt_rows<N, T>() * M * 16;   // rows >= Np are zero
   static constexpr int PT = 3 * Nfp * M * 16;
   static constexpr int I1D = 3 * Ng * Nfp;
   static constexpr int FM = (3 * Nfp + 1) / 2;   // ints packed into doubles
-  static constexpr int doubles = XT + 2 * PT + I1D + FM;
+  static constexpr int doubles = XT + PT + I1D + FM;
 };
+
+// Per-lane state of the cubature kernel's surface pass.
+struct FaceCtx {
+  int lg, el;
+  long ge;
+  int code[3];
+  double nx[3], ny[3], sj[3];
+  const double *XT, *PT, *I1;
+  const int *FM;
+  double tnow, lmax;
+  unsigned ef;
+};
+
+// Scaled numerical flux fsT = F* L/2 at face Gauss point kf = 4 ks + lg of
+// element el (zero past the last point): traces from the staged own and
+// neighbour face nodes, the exterior state on boundary faces.
+template <int N, int EQN, int FLUX, typename S>
+__device__ __forceinline__ void face_flux_cub(const RhsArgs &a, int ks, S *fsT, FaceCtx &F) {
+  constexpr int M = Vars<EQN>::M;
+  constexpr int Ng = Order<N>::Ng, Nfp = Order<N>::Nfp;
+  const int kf = 4 * ks + F.lg;
+  if (kf < 3 * Ng) {
+    const int f = kf / Ng, g = kf - f * Ng;
+    // selects of loaded values (a select of addresses would keep F in memory)
+    const int c0 = F.code[0], c1 = F.code[1], c2 = F.code[2];
+    const double x0 = F.nx[0], x1 = F.nx[1], x2 = F.nx[2];
+    const double y0 = F.ny[0], y1 = F.ny[1], y2 = F.ny[2];
+    const double s0 = F.sj[0], s1 = F.sj[1], s2 = F.sj[2];
+    const int code = f == 0 ? c0 : (f == 1 ? c1 : c2);
+    const double nx = f == 0 ? x0 : (f == 1 ? x1 : x2);
+    const double ny = f == 0 ? y0 : (f == 1 ? y1 : y2);
+    const double sj = f == 0 ? s0 : (f == 1 ? s1 : s2);
+    double qM[M], qP[M], fs[M];
+#pragma unroll
+    for (int v = 0; v < M; ++v) qM[v] = qP[v] = 0.0;
+    const double *Io = F.I1 + (f * Ng + g) * Nfp;
+#pragma unroll
+    for (int i = 0; i < Nfp; ++i) {
+      const double w = Io[i];
+      const double *xo = F.XT + F.FM[f * Nfp + i] * M * 16 + F.el;
+#pragma unroll
+      for (int v = 0; v < M; ++v) qM[v] += w * xo[v * 16];
+    }
+    if (code >= 0) {
+      // own node i <-> neighbour node Fmask[nf][Nfp-1-i]; the neighbour's
+      // trace at its Gauss point Ng-1-g is sum_j I1D[nf][Ng-1-g][j] q[Fmask[nf][j]]
+      const int nf = code & 3;
+      const double *In = F.I1 + (nf * Ng + (Ng - 1 - g)) * Nfp;
+#pragma unroll
+      for (int i = 0; i < Nfp; ++i) {
+        const double w = In[Nfp - 1 - i];
+#pragma unroll
+        for (int v = 0; v < M; ++v) qP[v] += w * F.PT[((f * Nfp + i) * M + v) * 16 + F.el];
+      }
+    } else {
+      const double *vk = a.vert + F.ge * 6;
+      const int va = f, vb = (f + 1) % 3;
+      const double z = a.I1D[3 * Ng * Nfp + g];
+      const double s = 0.5 * (1.0 + z);
+      const double xg = vk[2 * va] + s * (vk[2 * vb] - vk[2 * va]);
+      const double yg = vk[2 * va + 1] + s * (vk[2 * vb + 1] - vk[2 * va + 1]);
+      boundary_state<EQN>(a.bc, -code - 1, F.tnow, xg, yg, nx, ny, qM, qP);
+    }
+    double lam;
+    F.ef |= num_flux<EQN, FLUX>(a.ph, nx, ny, qM, qP, fs, &lam);
+    F.lmax = lam > F.lmax ? lam : F.lmax;
+#pragma unroll
+    for (int v = 0; v < M; ++v) fsT[v] = S(fs[v] * sj);
+  } else {
+#pragma unroll
+    for (int v = 0; v < M; ++v) fsT[v] = S(0);
+  }
+}
+
+// BF16x3 B operands of a K = Np product: row 32 kk + 4 it + lg of the staged
+// state (element el) split into hi/lo item it of step kk.
+template <int NK8, int M>
+__device__ __forceinline__ void split_state(const double *XT, int lg, int el, bf8 (&qh)[NK8][M],
+                                            bf8 (&ql)[NK8][M]) {
+#pragma unroll
+  for (int kk = 0; kk < NK8; ++kk)
+#pragma unroll
+    for (int it = 0; it < 8; ++it)
+#pragma unroll
+      for (int v = 0; v < M; ++v)
+        split_bf16(float(XT[((32 * kk + 4 * it + lg) * M + v) * 16 + el]), qh[kk][v], ql[kk][v], it);
+}
 
 // Waves per SIMD the register allocator is asked to allow: two where the
 // kernel fits 256 VGPRs without spilling (measured with
@@ -414,23 +556,27 @@ struct MinWaves {
 template <int N, int EQN, int FLUX, typename T>
 __global__ void __launch_bounds__(64, (MinWaves<N, EQN>::value)) rhs_kernel(RhsArgs a) {
   using O = Order<N>;
-  using L = Lds<N, EQN>;
+  using L = Lds<N, EQN, T>;
   constexpr int M = Vars<EQN>::M;
   constexpr int Np = O::Np, Nc = O::Nc, Ng = O::Ng, Nfp = O::Nfp;
   constexpr int NKS1 = O::NKS1, NCH = O::NCH, NRT = O::NRT, NKF = O::NKF;
   constexpr int ES = Np * M;  // doubles per element
   typedef typename Acc<T>::type V4;
+  typedef typename Scal<T>::type S;          // FP32 for split-BF16x3
+  constexpr bool BF = sizeof(T) == 1;
+  // 16x16x32 k-steps of GEMM1 (K = Np) and the face GEMM (K = 3 Ng): the
+  // k index 8 (lane/16) + item stands for the FP32 k-step 8 kk + item
+  constexpr int NK8 = (NKS1 + 7) / 8, NKF8 = (NKF + 7) / 8;
 
   extern __shared__ double lds[];
   double *XT = lds;
   double *PT = lds + L::XT;
-  double *QT = PT + L::PT;
-  double *I1 = QT + L::PT;
+  double *I1 = PT + L::PT;
   int *FM = (int *)(I1 + L::I1D);
 
   const int lane = threadIdx.x;
   const int el = lane & 15, lg = lane >> 4;
-  const long e0 = (long)blockIdx.x * 16;
+  const long e0 = (long)xcd_batch(blockIdx.x, gridDim.x) * 16;
   const int cnt = (int)min((long)16, (long)a.K - e0);
 
   // element data of this lane (element el; missing elements repeat element 0)
@@ -473,10 +619,7 @@ __global__ void __launch_bounds__(64, (MinWaves<N, EQN>::value)) rhs_kernel(RhsA
         for (int v = 0; v < M; ++v) v8[v] = 0.0;
       }
 #pragma unroll
-      for (int v = 0; v < M; ++v) {
-        PT[(k * M + v) * 16 + e] = v8[v];
-        QT[(k * M + v) * 16 + e] = XT[(FM[f * Nfp + i] * M + v) * 16 + e];
-      }
+      for (int v = 0; v < M; ++v) PT[(k * M + v) * 16 + e] = v8[v];
     }
   }
   __syncthreads();
@@ -488,136 +631,176 @@ __global__ void __launch_bounds__(64, (MinWaves<N, EQN>::value)) rhs_kernel(RhsA
     for (int v = 0; v < M; ++v) acc[rt][v] = V4{0, 0, 0, 0};
 
   unsigned ef = 0;
-  const T *A1 = (const T *)a.A1;
-  const T *A2 = (const T *)a.A2;
-  const T *A3 = (const T *)a.A3;
+  const S *A1 = (const S *)a.A1;
+  const S *A2 = (const S *)a.A2;
+  const S *A3 = (const S *)a.A3;
+  const Bf8x2 *B1 = (const Bf8x2 *)a.A1;   // BF16x3: [NCH][NK8][64]
+  const Bf8x2 *B2 = (const Bf8x2 *)a.A2;   // BF16x3: [NRT][NCH][64]
+  const Bf8x2 *B3 = (const Bf8x2 *)a.A3;   // BF16x3: [NRT][NKF8][64]
+
+  // BF16x3: B operands of GEMM1 (the state split into hi/lo), built once
+  // when they fit a handful of registers, else per chunk
+  constexpr bool HOIST = NK8 * M <= 8;
+  bf8 qh[NK8][M], ql[NK8][M];
+  if constexpr (BF && HOIST) split_state<NK8, M>(XT, lg, el, qh, ql);
 
   // 4. volume: chunks of 16 cubature points.  The A_r|A_s fragments of the
   // next chunk are loaded while the current one computes; GEMM1's operands
   // are loaded in one batch at the top of each chunk.
-  T a2n[NRT][8];
+  S a2n[BF ? 1 : NRT][8];
+  Bf8x2 f2n[BF ? NRT : 1];
 #pragma unroll
-  for (int rt = 0; rt < NRT; ++rt)
+  for (int rt = 0; rt < NRT; ++rt) {
+    if constexpr (BF) {
+      f2n[rt] = B2[(size_t)(rt * NCH) * 64 + lane];
+    } else {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) a2n[rt][j] = A2[((size_t)(rt * NCH) * 64 + lane) * 8 + j];
+      for (int j = 0; j < 8; ++j) a2n[rt][j] = A2[((size_t)(rt * NCH) * 64 + lane) * 8 + j];
+    }
+  }
 
 #pragma unroll 1
   for (int cc = 0; cc < NCH; ++cc) {
-    T a1[NKS1], b1[NKS1][M], a2[NRT][8];
-#pragma unroll
-    for (int ks = 0; ks < NKS1; ++ks) {
-      a1[ks] = A1[(cc * NKS1 + ks) * 64 + lane];
-#pragma unroll
-      for (int v = 0; v < M; ++v) b1[ks][v] = T(XT[((4 * ks + lg) * M + v) * 16 + el]);
-    }
-#pragma unroll
-    for (int rt = 0; rt < NRT; ++rt)
-#pragma unroll
-      for (int j = 0; j < 8; ++j) a2[rt][j] = a2n[rt][j];
-    if (cc + 1 < NCH) {
-#pragma unroll
-      for (int rt = 0; rt < NRT; ++rt)
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-          a2n[rt][j] = A2[((size_t)(rt * NCH + cc + 1) * 64 + lane) * 8 + j];
-    }
     V4 qc[M];
 #pragma unroll
     for (int v = 0; v < M; ++v) qc[v] = V4{0, 0, 0, 0};
+    S a2[BF ? 1 : NRT][8];
+    Bf8x2 f2[BF ? NRT : 1];
+    if constexpr (BF) {
+      if constexpr (!HOIST) split_state<NK8, M>(XT, lg, el, qh, ql);
 #pragma unroll
-    for (int ks = 0; ks < NKS1; ++ks)
+      for (int rt = 0; rt < NRT; ++rt) f2[rt] = f2n[rt];
+      if (cc + 1 < NCH) {
 #pragma unroll
-      for (int v = 0; v < M; ++v) qc[v] = mma(a1[ks], b1[ks][v], qc[v]);
-    // cubature point r of this lane: flux, geometric scaling, and its two
-    // rank-4 updates of the output tiles
+        for (int rt = 0; rt < NRT; ++rt) f2n[rt] = B2[(size_t)(rt * NCH + cc + 1) * 64 + lane];
+      }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int c = 16 * cc + acc_row<T>(lg, r);
-      T phr[M], phs[M];
-      if (c < Nc) {
-        T qv[M], F[M], G[M];
+      for (int kk = 0; kk < NK8; ++kk) {
+        const Bf8x2 f1 = B1[(cc * NK8 + kk) * 64 + lane];
 #pragma unroll
-        for (int v = 0; v < M; ++v) qv[v] = qc[v][r];
-        ef |= phys_flux<EQN, T>(a.ph, qv, F, G);
+        for (int v = 0; v < M; ++v) qc[v] = mma3(f1, qh[kk][v], ql[kk][v], qc[v]);
+      }
+    } else {
+      S a1[NKS1], b1[NKS1][M];
 #pragma unroll
-        for (int v = 0; v < M; ++v) {
-          phr[v] = T(jrx * double(F[v]) + jry * double(G[v]));
-          phs[v] = T(jsx * double(F[v]) + jsy * double(G[v]));
-        }
-      } else {
+      for (int ks = 0; ks < NKS1; ++ks) {
+        a1[ks] = A1[(cc * NKS1 + ks) * 64 + lane];
 #pragma unroll
-        for (int v = 0; v < M; ++v) phr[v] = phs[v] = T(0);
+        for (int v = 0; v < M; ++v) b1[ks][v] = S(XT[((4 * ks + lg) * M + v) * 16 + el]);
       }
 #pragma unroll
       for (int rt = 0; rt < NRT; ++rt)
 #pragma unroll
+        for (int j = 0; j < 8; ++j) a2[rt][j] = a2n[rt][j];
+      if (cc + 1 < NCH) {
+#pragma unroll
+        for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+            a2n[rt][j] = A2[((size_t)(rt * NCH + cc + 1) * 64 + lane) * 8 + j];
+      }
+#pragma unroll
+      for (int ks = 0; ks < NKS1; ++ks)
+#pragma unroll
+        for (int v = 0; v < M; ++v) qc[v] = mma(a1[ks], b1[ks][v], qc[v]);
+    }
+    // cubature point r of this lane: flux, geometric scaling, and its two
+    // rank-4 updates of the output tiles (BF16x3: items r and 4 + r of one
+    // 16x16x32 step)
+    bf8 gh[M], gl[M];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int c = 16 * cc + acc_row<T>(lg, r);
+      S phr[M], phs[M];
+      if (c < Nc) {
+        S qv[M], F[M], G[M];
+#pragma unroll
+        for (int v = 0; v < M; ++v) qv[v] = qc[v][r];
+        ef |= phys_flux<EQN, S>(a.ph, qv, F, G);
+#pragma unroll
         for (int v = 0; v < M; ++v) {
-          acc[rt][v] = mma(a2[rt][r], phr[v], acc[rt][v]);
-          acc[rt][v] = mma(a2[rt][4 + r], phs[v], acc[rt][v]);
+          phr[v] = S(jrx * double(F[v]) + jry * double(G[v]));
+          phs[v] = S(jsx * double(F[v]) + jsy * double(G[v]));
         }
+      } else {
+#pragma unroll
+        for (int v = 0; v < M; ++v) phr[v] = phs[v] = S(0);
+      }
+      if constexpr (BF) {
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+          split_bf16(phr[v], gh[v], gl[v], r);
+          split_bf16(phs[v], gh[v], gl[v], 4 + r);
+        }
+      } else {
+#pragma unroll
+        for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+          for (int v = 0; v < M; ++v) {
+            acc[rt][v] = mma(a2[rt][r], phr[v], acc[rt][v]);
+            acc[rt][v] = mma(a2[rt][4 + r], phs[v], acc[rt][v]);
+          }
+      }
+    }
+    if constexpr (BF) {
+#pragma unroll
+      for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+        for (int v = 0; v < M; ++v) acc[rt][v] = mma3(f2[rt], gh[v], gl[v], acc[rt][v]);
     }
   }
 
   // 5. surface: face Gauss point kf = 4 ks + lg of element el
-  const double tnow = (a.tptr ? *a.tptr : a.t) + a.tofs;
-  double lmax = 0.0;
+  FaceCtx F;
+  F.lg = lg;
+  F.el = el;
+  F.ge = ge;
+  F.code[0] = code0; F.code[1] = code1; F.code[2] = code2;
+  F.nx[0] = nx0; F.nx[1] = nx1; F.nx[2] = nx2;
+  F.ny[0] = ny0; F.ny[1] = ny1; F.ny[2] = ny2;
+  F.sj[0] = sj0; F.sj[1] = sj1; F.sj[2] = sj2;
+  F.XT = XT;
+  F.PT = PT;
+  F.I1 = I1;
+  F.FM = FM;
+  F.tnow = (a.tptr ? *a.tptr : a.t) + a.tofs;
+  F.lmax = 0.0;
+  F.ef = 0;
+  if constexpr (BF) {
 #pragma unroll 1
-  for (int ks = 0; ks < NKF; ++ks) {
-    const int kf = 4 * ks + lg;
-    T a3[NRT];
+    for (int kk = 0; kk < NKF8; ++kk) {
+      bf8 fh[M], fl[M];
 #pragma unroll
-    for (int rt = 0; rt < NRT; ++rt) a3[rt] = A3[(rt * NKF + ks) * 64 + lane];
-    T fsT[M];
-    if (kf < 3 * Ng) {
-      const int f = kf / Ng, g = kf - f * Ng;
-      const int code = f == 0 ? code0 : (f == 1 ? code1 : code2);
-      const double nx = f == 0 ? nx0 : (f == 1 ? nx1 : nx2);
-      const double ny = f == 0 ? ny0 : (f == 1 ? ny1 : ny2);
-      const double sj = f == 0 ? sj0 : (f == 1 ? sj1 : sj2);
-      double qM[M], qP[M], fs[M];
+      for (int it = 0; it < 8; ++it) {
+        S fsT[M];
+        face_flux_cub<N, EQN, FLUX, S>(a, 8 * kk + it, fsT, F);
 #pragma unroll
-      for (int v = 0; v < M; ++v) qM[v] = qP[v] = 0.0;
-      const double *Io = I1 + (f * Ng + g) * Nfp;
-#pragma unroll
-      for (int i = 0; i < Nfp; ++i) {
-        const double w = Io[i];
-#pragma unroll
-        for (int v = 0; v < M; ++v) qM[v] += w * QT[((f * Nfp + i) * M + v) * 16 + el];
+        for (int v = 0; v < M; ++v) split_bf16(fsT[v], fh[v], fl[v], it);
       }
-      if (code >= 0) {
-        // own node i <-> neighbour node Fmask[nf][Nfp-1-i]; the neighbour's
-        // trace at its Gauss point Ng-1-g is sum_j I1D[nf][Ng-1-g][j] q[Fmask[nf][j]]
-        const int nf = code & 3;
-        const double *In = I1 + (nf * Ng + (Ng - 1 - g)) * Nfp;
 #pragma unroll
-        for (int i = 0; i < Nfp; ++i) {
-          const double w = In[Nfp - 1 - i];
+      for (int rt = 0; rt < NRT; ++rt) {
+        const Bf8x2 f3 = B3[(rt * NKF8 + kk) * 64 + lane];
 #pragma unroll
-          for (int v = 0; v < M; ++v) qP[v] += w * PT[((f * Nfp + i) * M + v) * 16 + el];
-        }
-      } else {
-        const double *vk = a.vert + ge * 6;
-        const int va = f, vb = (f + 1) % 3;
-        const double z = a.I1D[3 * Ng * Nfp + g];
-        const double s = 0.5 * (1.0 + z);
-        const double xg = vk[2 * va] + s * (vk[2 * vb] - vk[2 * va]);
-        const double yg = vk[2 * va + 1] + s * (vk[2 * vb + 1] - vk[2 * va + 1]);
-        boundary_state<EQN>(a.bc, -code - 1, tnow, xg, yg, nx, ny, qM, qP);
+        for (int v = 0; v < M; ++v) acc[rt][v] = mma3(f3, fh[v], fl[v], acc[rt][v]);
       }
-      double lam;
-      ef |= num_flux<EQN, FLUX>(a.ph, nx, ny, qM, qP, fs, &lam);
-      lmax = lam > lmax ? lam : lmax;
-#pragma unroll
-      for (int v = 0; v < M; ++v) fsT[v] = T(fs[v] * sj);
-    } else {
-#pragma unroll
-      for (int v = 0; v < M; ++v) fsT[v] = T(0);
     }
+  } else {
+#pragma unroll 1
+    for (int ks = 0; ks < NKF; ++ks) {
+      S a3[NRT];
 #pragma unroll
-    for (int rt = 0; rt < NRT; ++rt)
+      for (int rt = 0; rt < NRT; ++rt) a3[rt] = A3[(rt * NKF + ks) * 64 + lane];
+      S fsT[M];
+      face_flux_cub<N, EQN, FLUX, S>(a, ks, fsT, F);
 #pragma unroll
-      for (int v = 0; v < M; ++v) acc[rt][v] = mma(a3[rt], fsT[v], acc[rt][v]);
+      for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+        for (int v = 0; v < M; ++v) acc[rt][v] = mma(a3[rt], fsT[v], acc[rt][v]);
+    }
   }
+
+  double lmax = F.lmax;
+  ef |= F.ef;
 
   // 6. epilogue: / J, stage through LDS as [e][n][v], coalesced write
   __syncthreads();
@@ -661,14 +844,14 @@ __global__ void __launch_bounds__(64, (MinWaves<N, EQN>::value)) rhs_kernel(RhsA
 //   R  -= Lt S,  S = F*(q-, q+) L/2 at the 3(N+1) face nodes
 //   rhs = R / J
 
-template <int N, int EQN>
+// LDS of the nodal kernel: only the transposed batch state (the own face
+// nodes are read from it through Fmask; the neighbours' face nodes go straight
+// to registers), so that LDS does not bound the number of resident waves.
+template <int N, int EQN, typename T>
 struct LinLds {
   static constexpr int M = Vars<EQN>::M;
-  static constexpr int Nfp = Order<N>::Nfp;
-  static constexpr int XT = Order<N>::NKS1 * 4 * M * 16;
-  static constexpr int PT = 3 * Nfp * M * 16;
-  static constexpr int FM = (3 * Nfp + 1) / 2;
-  static constexpr int doubles = XT + 2 * PT + FM;
+  static constexpr int XT = xt_rows<N, T>() * M * 16;
+  static constexpr int doubles = XT;
 };
 
 template <int N>
@@ -676,155 +859,225 @@ struct LinOrder {
   static constexpr int NKP = (3 * Order<N>::Nfp + 3) / 4;   // k-steps of the face GEMM
 };
 
+// The neighbour's values at face node kf = f Nfp + i of an element (its
+// node Fmask[nf][Nfp-1-i], given as fnb[nf]); zero on boundary faces and
+// past the last face node.
+template <int Np, int Nfp, int M>
+__device__ __forceinline__ void gather_face_node(const double *q, int kf, int code0, int code1,
+                                                 int code2, const int (&fnb)[3], double *dst) {
+  const int f = kf / Nfp;
+  const int code = f == 0 ? code0 : (f == 1 ? code1 : code2);
+  if (kf < 3 * Nfp && code >= 0) {
+    const int nf = code & 3;
+    const int node = nf == 0 ? fnb[0] : (nf == 1 ? fnb[1] : fnb[2]);
+    const double *src = q + ((long)(code >> 2) * Np + node) * M;
+#pragma unroll
+    for (int v = 0; v < M; ++v) dst[v] = src[v];
+  } else {
+#pragma unroll
+    for (int v = 0; v < M; ++v) dst[v] = 0.0;
+  }
+}
+
+// Waves per SIMD requested from the register allocator (measured with
+// -Rpass-analysis=kernel-resource-usage: no spills at these bounds).
+template <int N, int EQN, typename T>
+struct LinWaves {
+  static constexpr int value = EQN == APL_EQN_PHMAXWELL8 ? 1
+                              : sizeof(T) == 8 ? (N <= 6 ? 2 : 1)
+                                               : (N <= 6 ? 4 : 2);   // FP32 and BF16x3
+};
+
 template <int N, int EQN, int FLUX, typename T>
-__global__ void __launch_bounds__(64, (EQN == APL_EQN_PHMAXWELL8 ? 1 : 2)) rhs_linear_kernel(RhsArgs a) {
+__global__ void __launch_bounds__(64, (LinWaves<N, EQN, T>::value)) rhs_linear_kernel(RhsArgs a) {
   using O = Order<N>;
-  using L = LinLds<N, EQN>;
+  using L = LinLds<N, EQN, T>;
   constexpr int M = Vars<EQN>::M;
   constexpr int Np = O::Np, Nfp = O::Nfp;
   constexpr int NKS1 = O::NKS1, NRT = O::NRT, NKP = LinOrder<N>::NKP;
   constexpr int ES = Np * M;
   typedef typename Acc<T>::type V4;
+  typedef typename Scal<T>::type S;          // FP32 for split-BF16x3
+  constexpr bool BF = sizeof(T) == 1;
+  // 16x16x32 k-steps of the volume (K = Np) and face (K = 3 Nfp) products;
+  // k = 8 (lane/16) + item stands for the FP32 k-step 8 kk + item, so each
+  // lane's 8 items are the values it holds for eight 4-row steps
+  constexpr int NK8 = (NKS1 + 7) / 8, NKP8 = (NKP + 7) / 8;
 
   extern __shared__ double lds[];
   double *XT = lds;
-  double *PT = lds + L::XT;
-  double *QT = PT + L::PT;
-  int *FM = (int *)(QT + L::PT);
 
   const int lane = threadIdx.x;
   const int el = lane & 15, lg = lane >> 4;
-  const long e0 = (long)blockIdx.x * 16;
+  const long e0 = (long)xcd_batch(blockIdx.x, gridDim.x) * 16;
   const int cnt = (int)min((long)16, (long)a.K - e0);
 
   const long ge = e0 + (el < cnt ? el : 0);
   const double *gk = a.geom + ge * G_WORDS;
-  const double jrx = gk[G_JRX], jry = gk[G_JRY], jsx = gk[G_JSX], jsy = gk[G_JSY];
-  const double Jel = gk[G_J];
-  const double nx0 = gk[G_NX], nx1 = gk[G_NX + 1], nx2 = gk[G_NX + 2];
-  const double ny0 = gk[G_NY], ny1 = gk[G_NY + 1], ny2 = gk[G_NY + 2];
-  const double sj0 = gk[G_SJ], sj1 = gk[G_SJ + 1], sj2 = gk[G_SJ + 2];
   const int code0 = a.fcode[ge * 3], code1 = a.fcode[ge * 3 + 1], code2 = a.fcode[ge * 3 + 2];
 
-  for (int i = lane; i < 3 * Nfp; i += 64) FM[i] = a.fmask[i];
+  // face node kf = 4 j + lg of element el: own node Fmask[f][i] and the
+  // neighbour's node Fmask[nf][Nfp-1-i] for each possible neighbour face nf
+  int fown[NKP], fnb[NKP][3];
+#pragma unroll
+  for (int j = 0; j < NKP; ++j) {
+    const int kf = 4 * j + lg;
+    const int f = kf < 3 * Nfp ? kf / Nfp : 0, i = kf < 3 * Nfp ? kf - f * Nfp : 0;
+    fown[j] = a.fmask[f * Nfp + i];
+#pragma unroll
+    for (int nf = 0; nf < 3; ++nf) fnb[j][nf] = a.fmask[nf * Nfp + (Nfp - 1 - i)];
+  }
+
+  // Neighbour face-node values (zero on boundary faces).  Up to 5 variables
+  // they are all loaded into registers before the batch staging, so their
+  // latency overlaps it; phMaxwell loads them per k-step.
+  const double *qg = a.q;
+  constexpr bool PRE = M <= 5;
+  double qP[PRE ? NKP : 1][M];
+  if constexpr (PRE) {
+#pragma unroll
+    for (int j = 0; j < NKP; ++j)
+      gather_face_node<Np, Nfp, M>(qg, 4 * j + lg, code0, code1, code2, fnb[j], qP[j]);
+  }
+
   stage_transposed<ES>(XT, a.q + e0 * ES, cnt, lane);
   for (int i = 16 * ES + lane; i < L::XT; i += 64) XT[i] = 0.0;
   __syncthreads();
 
-  // own and neighbour face nodes, in the own face's node order
-  constexpr int NGI = (16 * 3 * Nfp + 63) / 64;
-#pragma unroll
-  for (int j = 0; j < NGI; ++j) {
-    const int it = lane + 64 * j;
-    if (it < 16 * 3 * Nfp) {
-      const int e = it & 15, k = it >> 4, f = k / Nfp, i = k - f * Nfp;
-      const long gee = e0 + (e < cnt ? e : 0);
-      const int code = a.fcode[gee * 3 + f];
-      double v8[M];
-      if (code >= 0) {
-        const long nb = code >> 2;
-        const int nf = code & 3;
-        const double *src = a.q + (nb * Np + FM[nf * Nfp + (Nfp - 1 - i)]) * M;
-#pragma unroll
-        for (int v = 0; v < M; ++v) v8[v] = src[v];
-      } else {
-#pragma unroll
-        for (int v = 0; v < M; ++v) v8[v] = 0.0;
-      }
-#pragma unroll
-      for (int v = 0; v < M; ++v) {
-        PT[(k * M + v) * 16 + e] = v8[v];
-        QT[(k * M + v) * 16 + e] = XT[(FM[f * Nfp + i] * M + v) * 16 + e];
-      }
-    }
-  }
-  __syncthreads();
-
   unsigned ef = 0;
-  const T *A1 = (const T *)a.A1;   // [2][NRT][NKS1][64]: Dt_r, Dt_s
-  const T *A3 = (const T *)a.A3;   // [NRT][NKP][64]: -Lt
-
-  // volume at the nodes
-  T b1[NKS1][M];
-#pragma unroll
-  for (int ks = 0; ks < NKS1; ++ks)
-#pragma unroll
-    for (int v = 0; v < M; ++v) b1[ks][v] = T(XT[((4 * ks + lg) * M + v) * 16 + el]);
+  const S *A1 = (const S *)a.A1;   // [2][NRT][NKS1][64]: Dt_r, Dt_s
+  const S *A3 = (const S *)a.A3;   // [NRT][NKP][64]: -Lt
+  const Bf8x2 *B1 = (const Bf8x2 *)a.A1;   // BF16x3: [2][NRT][NK8][64]
+  const Bf8x2 *B3 = (const Bf8x2 *)a.A3;   // BF16x3: [NRT][NKP8][64]
+  bf8 fh[M], fl[M];                          // BF16x3: face fluxes of one k-step
 
   V4 acc[NRT][M];
 #pragma unroll
-  for (int rt = 0; rt < NRT; ++rt) {
-    T ar[NKS1], as[NKS1];
+  for (int rt = 0; rt < NRT; ++rt)
 #pragma unroll
-    for (int ks = 0; ks < NKS1; ++ks) {
-      ar[ks] = A1[((0 * NRT + rt) * NKS1 + ks) * 64 + lane];
-      as[ks] = A1[((1 * NRT + rt) * NKS1 + ks) * 64 + lane];
-    }
-    V4 pr[M], ps[M];
-#pragma unroll
-    for (int v = 0; v < M; ++v) pr[v] = ps[v] = V4{0, 0, 0, 0};
-#pragma unroll
-    for (int ks = 0; ks < NKS1; ++ks)
-#pragma unroll
-      for (int v = 0; v < M; ++v) {
-        pr[v] = mma(ar[ks], b1[ks][v], pr[v]);
-        ps[v] = mma(as[ks], b1[ks][v], ps[v]);
-      }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      T p1[M], p2[M], F1[M], G1[M], F2[M], G2[M];
-#pragma unroll
-      for (int v = 0; v < M; ++v) {
-        p1[v] = pr[v][r];
-        p2[v] = ps[v][r];
-      }
-      phys_flux<EQN, T>(a.ph, p1, F1, G1);
-      phys_flux<EQN, T>(a.ph, p2, F2, G2);
-#pragma unroll
-      for (int v = 0; v < M; ++v)
-        acc[rt][v][r] = T(jrx * double(F1[v]) + jry * double(G1[v]) + jsx * double(F2[v]) +
-                          jsy * double(G2[v]));
-    }
-  }
+    for (int v = 0; v < M; ++v) acc[rt][v] = V4{0, 0, 0, 0};
 
-  // surface at the face nodes kf = 4 ks + lg
+  // surface at the face nodes kf = 4 ks + lg: R = -Lt S
   double lmax = 0.0;
-#pragma unroll 1
+#pragma unroll
   for (int ks = 0; ks < NKP; ++ks) {
     const int kf = 4 * ks + lg;
-    T a3[NRT];
+    S a3[NRT];
+    if constexpr (!BF) {
 #pragma unroll
-    for (int rt = 0; rt < NRT; ++rt) a3[rt] = A3[(rt * NKP + ks) * 64 + lane];
-    T fsT[M];
+      for (int rt = 0; rt < NRT; ++rt) a3[rt] = A3[(rt * NKP + ks) * 64 + lane];
+    }
+    S fsT[M];
     if (kf < 3 * Nfp) {
       const int f = kf / Nfp;
       const int code = f == 0 ? code0 : (f == 1 ? code1 : code2);
-      const double nx = f == 0 ? nx0 : (f == 1 ? nx1 : nx2);
-      const double ny = f == 0 ? ny0 : (f == 1 ? ny1 : ny2);
-      const double sj = f == 0 ? sj0 : (f == 1 ? sj1 : sj2);
-      double qM[M], qP[M], fs[M];
+      const double nx = gk[G_NX + f], ny = gk[G_NY + f], sj = gk[G_SJ + f];
+      double qM[M], fs[M];
+      double *qp = qP[PRE ? ks : 0];
+      if constexpr (!PRE) gather_face_node<Np, Nfp, M>(qg, kf, code0, code1, code2, fnb[ks], qp);
 #pragma unroll
-      for (int v = 0; v < M; ++v) {
-        qM[v] = QT[(kf * M + v) * 16 + el];
-        qP[v] = PT[(kf * M + v) * 16 + el];
-      }
-      if (code < 0) boundary_state<EQN>(a.bc, -code - 1, 0.0, 0.0, 0.0, nx, ny, qM, qP);
+      for (int v = 0; v < M; ++v) qM[v] = XT[(fown[ks] * M + v) * 16 + el];
+      if (code < 0) boundary_state<EQN>(a.bc, -code - 1, 0.0, 0.0, 0.0, nx, ny, qM, qp);
       double lam;
-      ef |= num_flux<EQN, FLUX>(a.ph, nx, ny, qM, qP, fs, &lam);
+      ef |= num_flux<EQN, FLUX>(a.ph, nx, ny, qM, qp, fs, &lam);
       lmax = lam > lmax ? lam : lmax;
 #pragma unroll
-      for (int v = 0; v < M; ++v) fsT[v] = T(fs[v] * sj);
+      for (int v = 0; v < M; ++v) fsT[v] = S(fs[v] * sj);
     } else {
 #pragma unroll
-      for (int v = 0; v < M; ++v) fsT[v] = T(0);
+      for (int v = 0; v < M; ++v) fsT[v] = S(0);
     }
+    if constexpr (BF) {
+      const int it = ks & 7;
 #pragma unroll
-    for (int rt = 0; rt < NRT; ++rt)
+      for (int v = 0; v < M; ++v) split_bf16(fsT[v], fh[v], fl[v], it);
+      if (it == 7 || ks == NKP - 1) {
 #pragma unroll
-      for (int v = 0; v < M; ++v) acc[rt][v] = mma(a3[rt], fsT[v], acc[rt][v]);
+        for (int j = it + 1; j < 8; ++j)
+#pragma unroll
+          for (int v = 0; v < M; ++v) fh[v][j] = fl[v][j] = (__bf16)0.0f;
+#pragma unroll
+        for (int rt = 0; rt < NRT; ++rt) {
+          const Bf8x2 f3 = B3[(rt * NKP8 + ks / 8) * 64 + lane];
+#pragma unroll
+          for (int v = 0; v < M; ++v) acc[rt][v] = mma3(f3, fh[v], fl[v], acc[rt][v]);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+        for (int v = 0; v < M; ++v) acc[rt][v] = mma(a3[rt], fsT[v], acc[rt][v]);
+    }
+  }
+
+  // volume at the nodes: P_r = Dt_r Q then P_s = Dt_s Q, each mixed by the
+  // flux matrices and the geometric factors straight into the accumulator
+  const double jrx = gk[G_JRX], jry = gk[G_JRY], jsx = gk[G_JSX], jsy = gk[G_JSY];
+  // B operands: held in registers for FP32; re-read from LDS per k-step for
+  // FP64, whose doubled register footprint would otherwise cost a wave/SIMD
+  constexpr bool KEEP_B = sizeof(T) == 4;
+  S b1[KEEP_B ? NKS1 : 1][M];
+  if constexpr (KEEP_B) {
+#pragma unroll
+    for (int ks = 0; ks < NKS1; ++ks)
+#pragma unroll
+      for (int v = 0; v < M; ++v) b1[ks][v] = S(XT[((4 * ks + lg) * M + v) * 16 + el]);
+  }
+  // BF16x3: the state split once into hi/lo B operands
+  bf8 bh[BF ? NK8 : 1][M], bl[BF ? NK8 : 1][M];
+  if constexpr (BF) {
+#pragma unroll
+    for (int kk = 0; kk < NK8; ++kk)
+#pragma unroll
+      for (int it = 0; it < 8; ++it)
+#pragma unroll
+        for (int v = 0; v < M; ++v)
+          split_bf16(float(XT[((32 * kk + 4 * it + lg) * M + v) * 16 + el]), bh[kk][v], bl[kk][v], it);
+  }
+
+#pragma unroll
+  for (int rt = 0; rt < NRT; ++rt) {
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      V4 p[M];
+#pragma unroll
+      for (int v = 0; v < M; ++v) p[v] = V4{0, 0, 0, 0};
+      if constexpr (BF) {
+#pragma unroll
+        for (int kk = 0; kk < NK8; ++kk) {
+          const Bf8x2 f1 = B1[((d * NRT + rt) * NK8 + kk) * 64 + lane];
+#pragma unroll
+          for (int v = 0; v < M; ++v) p[v] = mma3(f1, bh[kk][v], bl[kk][v], p[v]);
+        }
+      } else {
+        S ad[NKS1];
+#pragma unroll
+        for (int ks = 0; ks < NKS1; ++ks) ad[ks] = A1[((d * NRT + rt) * NKS1 + ks) * 64 + lane];
+#pragma unroll
+        for (int ks = 0; ks < NKS1; ++ks)
+#pragma unroll
+          for (int v = 0; v < M; ++v) {
+            const S b = KEEP_B ? b1[KEEP_B ? ks : 0][v] : S(XT[((4 * ks + lg) * M + v) * 16 + el]);
+            p[v] = mma(ad[ks], b, p[v]);
+          }
+      }
+      const double cx = d == 0 ? jrx : jsx, cy = d == 0 ? jry : jsy;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        S pv[M], F[M], G[M];
+#pragma unroll
+        for (int v = 0; v < M; ++v) pv[v] = p[v][r];
+        phys_flux<EQN, S>(a.ph, pv, F, G);
+#pragma unroll
+        for (int v = 0; v < M; ++v)
+          acc[rt][v][r] += S(cx * double(F[v]) + cy * double(G[v]));
+      }
+    }
   }
 
   // epilogue: / J, stage through LDS as [e][n][v], coalesced write
+  const double Jel = gk[G_J];
   __syncthreads();
   double *OUT = lds;
 #pragma unroll

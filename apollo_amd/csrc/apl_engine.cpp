@@ -167,7 +167,8 @@ static int check_flags(apl_handle *h) {
 }
 
 // Fold and permute the operators into MFMA fragments (see apl_device.h).
-template <typename T>
+// LT is the operand type whose accumulator layout the fragments follow.
+template <typename T, typename LT = T>
 static void make_fragments(const Operators &o, std::vector<T> &A1, std::vector<T> &A2,
                            std::vector<T> &A3) {
   const int Np = o.Np, Nc = o.Nc, Ng = o.Ng;
@@ -202,7 +203,7 @@ static void make_fragments(const Operators &o, std::vector<T> &A1, std::vector<T
     for (int cc = 0; cc < NCH; ++cc)
       for (int l = 0; l < 64; ++l)
         for (int r = 0; r < 4; ++r) {
-          const int n = 16 * rt + (l & 15), c = 16 * cc + acc_row<T>(l >> 4, r);
+          const int n = 16 * rt + (l & 15), c = 16 * cc + acc_row<LT>(l >> 4, r);
           if (n < Np && c < Nc) {
             const size_t b = (((size_t)rt * NCH + cc) * 64 + l) * 8;
             A2[b + r] = T(Ar[(size_t)n * Nc + c]);
@@ -292,6 +293,81 @@ static int upload_fragments(apl_handle *h) {
   if ((rc = dev_upload<T>(h, (T **)&h->d_A1, A1.data(), A1.size()))) return rc;
   if ((rc = dev_upload<T>(h, (T **)&h->d_A2, A2.data(), A2.size()))) return rc;
   if ((rc = dev_upload<T>(h, (T **)&h->d_A3, A3.data(), A3.size()))) return rc;
+  return APL_OK;
+}
+
+// BF16 (bfloat16) bits of x, rounded to nearest even; NaN and Inf pass.
+static uint16_t bf16_rne(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  if ((u & 0x7f800000u) != 0x7f800000u) u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+static float bf16_value(uint16_t b) {
+  const uint32_t u = (uint32_t)b << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+// One lane's split-BF16 fragment of a 16x16x32 step: 8 hi then 8 lo values
+// (the layout of Bf8x2 in apl_device.h).
+struct Bf16Pair {
+  uint16_t v[16];
+};
+
+static_assert(sizeof(Bf16Pair) == sizeof(apl::Bf8x2), "one lane's hi/lo fragment is 32 bytes");
+
+static void split_into(double x, Bf16Pair &p, int item) {
+  const float xf = (float)x;
+  const uint16_t hi = bf16_rne(xf);
+  p.v[item] = hi;
+  p.v[8 + item] = bf16_rne(xf - bf16_value(hi));
+}
+
+// FP32-layout fragments [outer][nks][64] (one value per lane per 4-row
+// k-step) regrouped into 16x16x32 steps [outer][(nks+7)/8][64]: item i of
+// step kk is the value of k-step 8 kk + i (zero past nks).
+static std::vector<Bf16Pair> regroup_bf16(const std::vector<double> &F, int outer, int nks) {
+  const int nk8 = (nks + 7) / 8;
+  std::vector<Bf16Pair> out((size_t)outer * nk8 * 64);
+  for (auto &p : out) std::memset(p.v, 0, sizeof(p.v));
+  for (int o = 0; o < outer; ++o)
+    for (int ks = 0; ks < nks; ++ks)
+      for (int l = 0; l < 64; ++l)
+        split_into(F[((size_t)o * nks + ks) * 64 + l], out[((size_t)o * nk8 + ks / 8) * 64 + l], ks % 8);
+  return out;
+}
+
+// Split-BF16x3 fragments: the FP32 fragments (FP32 accumulator layout) in
+// FP64, regrouped per 16x16x32 step and split into hi/lo.
+static int upload_fragments_bf16x3(apl_handle *h) {
+  const Operators &o = h->ops;
+  const int Np = o.Np, Nc = o.Nc, Ng = o.Ng, Nfp = o.Nfp;
+  const int NKS1 = (Np + 3) / 4, NCH = (Nc + 15) / 16, NRT = (Np + 15) / 16;
+  std::vector<double> A1, A2, A3;
+  std::vector<Bf16Pair> B1, B2, B3;
+  if (h->linear) {
+    make_linear_fragments<double>(o, A1, A3);
+    B1 = regroup_bf16(A1, 2 * NRT, NKS1);
+    B3 = regroup_bf16(A3, NRT, (3 * Nfp + 3) / 4);
+    B2.resize(1);
+    std::memset(B2[0].v, 0, sizeof(B2[0].v));
+  } else {
+    make_fragments<double, float>(o, A1, A2, A3);
+    B1 = regroup_bf16(A1, NCH, NKS1);
+    // A_r|A_s fragments already hold 8 values per lane: items 0-3 (A_r) and
+    // 4-7 (A_s) of one step
+    B2.resize((size_t)NRT * NCH * 64);
+    for (size_t i = 0; i < B2.size(); ++i)
+      for (int it = 0; it < 8; ++it) split_into(A2[i * 8 + it], B2[i], it);
+    B3 = regroup_bf16(A3, NRT, (3 * Ng + 3) / 4);
+  }
+  int rc;
+  if ((rc = dev_upload<Bf16Pair>(h, (Bf16Pair **)&h->d_A1, B1.data(), B1.size()))) return rc;
+  if ((rc = dev_upload<Bf16Pair>(h, (Bf16Pair **)&h->d_A2, B2.data(), B2.size()))) return rc;
+  if ((rc = dev_upload<Bf16Pair>(h, (Bf16Pair **)&h->d_A3, B3.data(), B3.size()))) return rc;
   return APL_OK;
 }
 
@@ -494,7 +570,8 @@ int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out)
   if ((rc = dev_upload<int>(h, &h->d_fmask, o.fmask.data(), o.fmask.size()))) return bail(rc);
   if ((rc = dev_upload<double>(h, &h->d_I1D, I1D.data(), I1D.size()))) return bail(rc);
   if (params->precision == APL_PREC_FP64) rc = upload_fragments<double>(h);
-  else rc = upload_fragments<float>(h);
+  else if (params->precision == APL_PREC_FP32_MFMA) rc = upload_fragments<float>(h);
+  else rc = upload_fragments_bf16x3(h);
   if (rc) return bail(rc);
   if ((rc = dev_alloc(h, (void **)&h->d_err, sizeof(unsigned) * 4))) return bail(rc);
   if ((rc = dev_alloc(h, (void **)&h->d_speed, sizeof(double) * ((h->K + 15) / 16)))) return bail(rc);
@@ -596,8 +673,6 @@ int apl_host_connectivity(const apl_mesh *mesh, int *etetf) {
 int apl_rhs(apl_handle *h, double t, const double *q, double *rhs, double cfl,
             double *suggested_dt, int where) {
   if (!h || !q || !rhs) return APL_ERR_ARG;
-  if (h->params.precision == APL_PREC_BF16X3)
-    return fail(h, APL_ERR_UNSUPPORTED, "BF16X3 RHS not built yet");
   const size_t bytes = sizeof(double) * state_len(h);
   int rc;
   const double *dq = q;
@@ -710,11 +785,20 @@ static int enqueue_step(apl_handle *h, double *src, double *dst, double dt, bool
   return APL_OK;
 }
 
+// Whether a kernel reads the time: only the isentropic-vortex boundary
+// state depends on it.  Otherwise the captured step skips the clock kernel.
+static bool time_dependent(const apl_handle *h) {
+  for (int i = 0; i < h->params.n_bc; ++i)
+    if (h->params.bc[i].kind == APL_BC_ISENTROPIC_VORTEX) return true;
+  return false;
+}
+
 static int build_graphs(apl_handle *h, double *qa, double *qb, double dt) {
   free_graphs(h);
+  const bool adv = time_dependent(h);
   for (int i = 0; i < 2; ++i) {
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    int rc = (i == 0) ? enqueue_step(h, qa, qb, dt, true) : enqueue_step(h, qb, qa, dt, true);
+    int rc = (i == 0) ? enqueue_step(h, qa, qb, dt, adv) : enqueue_step(h, qb, qa, dt, adv);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(h->stream, &g);
     if (rc) {
@@ -746,8 +830,6 @@ static int run_lserk4(apl_handle *h, double t0, double dt, int nsteps, double *q
 
 int apl_lserk4(apl_handle *h, double t0, double dt, int nsteps, double *q, int where) {
   if (!h || !q || nsteps < 0) return APL_ERR_ARG;
-  if (h->params.precision == APL_PREC_BF16X3)
-    return fail(h, APL_ERR_UNSUPPORTED, "BF16X3 RHS not built yet");
   if (nsteps == 0) return APL_OK;
   int rc;
   if ((rc = ensure_work(h))) return rc;

@@ -14,7 +14,7 @@ template <int N, int EQN, int FLUX, typename T>
 static int launch_typed(int linear, hipStream_t s, const RhsArgs &a, int nblocks) {
   if constexpr (EQN == APL_EQN_PHMAXWELL8 || EQN == APL_EQN_MAXWELL_TM3) {
     if (linear) {
-      const size_t lds = sizeof(double) * LinLds<N, EQN>::doubles;
+      const size_t lds = sizeof(double) * LinLds<N, EQN, T>::doubles;
       auto kern = rhs_linear_kernel<N, EQN, FLUX, T>;
       if (lds > 65536 && hipFuncSetAttribute((const void *)kern,
                                              hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -25,7 +25,7 @@ static int launch_typed(int linear, hipStream_t s, const RhsArgs &a, int nblocks
     }
   }
   if (linear) return APL_ERR_ARG;
-  const size_t lds = sizeof(double) * Lds<N, EQN>::doubles;
+  const size_t lds = sizeof(double) * Lds<N, EQN, T>::doubles;
   auto kern = rhs_kernel<N, EQN, FLUX, T>;
   if (lds > 65536) {
     if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -41,6 +41,7 @@ static int launch_prec(int prec, int linear, hipStream_t s, const RhsArgs &a, in
   switch (prec) {
     case APL_PREC_FP64: return launch_typed<N, EQN, FLUX, double>(linear, s, a, nblocks);
     case APL_PREC_FP32_MFMA: return launch_typed<N, EQN, FLUX, float>(linear, s, a, nblocks);
+    case APL_PREC_BF16X3: return launch_typed<N, EQN, FLUX, Bf16x3>(linear, s, a, nblocks);
     default: return APL_ERR_UNSUPPORTED;
   }
 }

@@ -99,6 +99,51 @@ def test_fp32_mfma_rhs(N):
         assert err <= 2e-5
 
 
+BF16X3_TOL = 1e-3
+
+
+@pytest.mark.parametrize("N", [1, 3, 4, 5, 8])
+def test_bf16x3_rhs(N):
+    """Split-BF16x3 contractions: each operand is split as x = hi + lo with
+    hi = bf16(x), lo = bf16(x - hi), so |x - hi - lo| <= 2^-18 |x|, and the
+    dropped lo.lo term is <= 2^-18 |ab|: operand rounding is 2^6 times that
+    of FP32 (2^-24).  The RHS error carries the operators' cancellation
+    factor in both variants, so the bar is stated against the FP32-MFMA
+    error of the same case: BF16x3 <= 2^8 x FP32 (one BF16 product alone
+    would be ~2^15 x), and <= 1e-3 relative L2 against the FP64 oracle."""
+    cases = (euler_case(n=6, N=N, m=4), euler_case(n=5, N=N, m=4, periodic=False),
+             maxwell_case(n=5, N=N, m=3, bc=E.BC_PEC, flux=E.FLUX_UPWIND))
+    for case in cases:
+        q = case.state()
+        ref, _ = case.oracle_rhs(q)
+        algos = [None] if case.name == "euler" else [E.ALGO_AUTO, E.ALGO_CUBATURE]
+        for algo in algos:
+            e32 = rel_l2(_rhs(case, q, precision=E.PREC_FP32_MFMA, algorithm=algo), ref)
+            ebf = rel_l2(_rhs(case, q, precision=E.PREC_BF16X3, algorithm=algo), ref)
+            print("N=%d %s algo=%s rel L2: FP32 %.3e BF16x3 %.3e (ratio %.0f)"
+                  % (N, case.name, algo, e32, ebf, ebf / e32))
+            assert ebf <= BF16X3_TOL
+            assert ebf <= 2.0 ** 8 * e32
+
+
+def test_bf16x3_lserk4_graph():
+    """LSERK4 steps with BF16x3 contractions against the FP64 engine."""
+    for case in (euler_case(n=10, N=4, m=4), maxwell_case(n=8, N=5, m=3, bc=E.BC_PEC)):
+        q0 = case.state()
+        out = {}
+        for prec in (E.PREC_FP64, E.PREC_BF16X3):
+            p = case.params
+            eng = E.Engine(case.mesh, E.Params(p.order, p.eqn, flux=p.flux, precision=prec,
+                                               bcs=p.bcs))
+            q = q0.copy()
+            eng.lserk4(0.0, 0.002, 10, q)
+            out[prec] = q
+            eng.close()
+        err = rel_l2(out[E.PREC_BF16X3] - q0, out[E.PREC_FP64] - q0)
+        print("BF16x3 %s LSERK4 increment rel L2 vs FP64 %.3e" % (case.name, err))
+        assert err <= BF16X3_TOL
+
+
 def test_ragged_gmsh_mesh():
     """250-triangle Gmsh mesh from Apollo's vortex deck: K not a multiple of 16."""
     from apollo_amd.gmsh import read_gmsh
