@@ -207,6 +207,65 @@ __device__ __forceinline__ void boundary_vjp(const BCTable &B, int idx, double n
   for (int c = 0; c < M; ++c) qMb[c] += t[c];
 }
 
+// y[i] += b x[i] over one batch of 16 elements of ES doubles, written back,
+// and the updated values staged transposed as stage_transposed does
+// (XT[w*16 + e]; missing elements repeat element 0).
+template <int ES>
+__device__ __forceinline__ void axpy_stage_transposed(double *XT, double *y, const double *x,
+                                                      double b, int cnt, int lane) {
+  constexpr int NV2 = 16 * ES / 2, NIT = (NV2 + 63) / 64;
+  if (cnt == 16) {
+    dbl2 yv[NIT], xv[NIT];
+#pragma unroll
+    for (int j = 0; j < NIT; ++j) {
+      const int idx = lane + 64 * j;
+      if (idx < NV2) {
+        yv[j] = ((const dbl2 *)y)[idx];
+        xv[j] = ((const dbl2 *)x)[idx];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NIT; ++j) {
+      const int idx = lane + 64 * j;
+      if (idx < NV2) {
+        const dbl2 v = yv[j] + b * xv[j];
+        ((dbl2 *)y)[idx] = v;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int i = 2 * idx + h, e = i / ES, w = i - e * ES;
+          XT[w * 16 + e] = v[h];
+        }
+      }
+    }
+  } else {
+    for (int i = lane; i < 16 * ES; i += 64) {
+      const int e = i / ES, w = i - e * ES;
+      if (e < cnt) {
+        const double v = y[i] + b * x[i];
+        y[i] = v;
+        XT[w * 16 + e] = v;
+      } else {
+        XT[w * 16 + e] = 0.0;
+      }
+    }
+  }
+}
+
+// dst[i] = src[i] (LDS to global) for cnt elements of ES doubles each.
+template <int ES>
+__device__ __forceinline__ void copy_batch(double *dst, const double *src, int cnt, int lane) {
+  constexpr int NV2 = 16 * ES / 2, NIT = (NV2 + 63) / 64;
+  if (cnt == 16) {
+#pragma unroll
+    for (int j = 0; j < NIT; ++j) {
+      const int idx = lane + 64 * j;
+      if (idx < NV2) ((dbl2 *)dst)[idx] = dbl2{src[2 * idx], src[2 * idx + 1]};
+    }
+  } else {
+    for (int i = lane; i < cnt * ES; i += 64) dst[i] = src[i];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // kernel A: lh = lam / J (mode 1: rbar += b qbar first), Sb = -L^T lh
 // ---------------------------------------------------------------------------
@@ -220,41 +279,44 @@ struct AdjFaceLds {
 };
 
 template <int N, int EQN>
-__global__ void __launch_bounds__(64) adj_face_kernel(AdjArgs a) {
+struct AdjFaceGroup {
+  static constexpr int W = group_waves(AdjFaceLds<N, EQN>::doubles, 8);
+};
+
+template <int N, int EQN>
+__global__ void __launch_bounds__((64 * AdjFaceGroup<N, EQN>::W)) adj_face_kernel(AdjArgs a) {
   using O = Order<N>;
   constexpr int M = Vars<EQN>::M;
   constexpr int Np = O::Np, Ng = O::Ng, NKS1 = O::NKS1, NFT = AdjOrder<N>::NFT;
   constexpr int ES = Np * M, FS = 3 * Ng * M;
   typedef typename Acc<double>::type V4;
-  extern __shared__ double lds[];
+  constexpr int W = AdjFaceGroup<N, EQN>::W;
+  const WaveBatch wb = wave_batch<W>(a.K);
+  if (!wb.active) return;
+  extern __shared__ double lds_all[];
+  double *lds = lds_all + wb.wave * AdjFaceLds<N, EQN>::doubles;
   double *LT = lds;
   double *SO = lds + AdjFaceLds<N, EQN>::LT;
 
-  const int lane = threadIdx.x, el = lane & 15, lg = lane >> 4;
-  const long e0 = (long)xcd_batch(blockIdx.x, gridDim.x) * 16;
-  const int cnt = (int)min((long)16, (long)a.K - e0);
+  const int lane = wb.lane, el = lane & 15, lg = lane >> 4;
+  const long e0 = wb.e0;
+  const int cnt = wb.cnt;
   double *rb = (double *)a.lam;
-  for (int i = lane; i < 16 * ES; i += 64) {
-    const int e = i / ES, w = i - e * ES;
-    double v = 0.0;
-    if (e < cnt) {
-      const long gi = e0 * ES + i;
-      v = rb[gi];
-      if (a.mode == 1) {
-        v += a.rk_b * a.qbar[gi];
-        rb[gi] = v;
-      }
-      v /= a.geom[(e0 + e) * G_WORDS + G_J];
-    }
-    LT[w * 16 + e] = v;
-  }
+  // mode 1: rbar += b qbar (written back); LT = rbar transposed.  1/J is
+  // applied to the B operands (lane el's own element; 0 past the batch end).
+  const long ge = e0 + (el < cnt ? el : 0);
+  const double sc = el < cnt ? a.geom[ge * G_WORDS + G_INVJ] : 0.0;
+  if (a.mode == 1 && wb.active)
+    axpy_stage_transposed<ES>(LT, rb + e0 * ES, a.qbar + e0 * ES, a.rk_b, cnt, lane);
+  else
+    stage_transposed<ES>(LT, rb + e0 * ES, cnt, lane);
   for (int i = 16 * ES + lane; i < NKS1 * 4 * M * 16; i += 64) LT[i] = 0.0;
-  __syncthreads();
+  wave_sync();
   double b[NKS1][M];
 #pragma unroll
   for (int ks = 0; ks < NKS1; ++ks)
 #pragma unroll
-    for (int v = 0; v < M; ++v) b[ks][v] = LT[((4 * ks + lg) * M + v) * 16 + el];
+    for (int v = 0; v < M; ++v) b[ks][v] = LT[((4 * ks + lg) * M + v) * 16 + el] * sc;
 #pragma unroll
   for (int ft = 0; ft < NFT; ++ft) {
     V4 acc[M];
@@ -275,13 +337,63 @@ __global__ void __launch_bounds__(64) adj_face_kernel(AdjArgs a) {
       }
     }
   }
-  __syncthreads();
-  for (int i = lane; i < cnt * FS; i += 64) a.sbar[e0 * FS + i] = SO[i];
+  wave_sync();
+  if (wb.active) copy_batch<FS>(a.sbar + e0 * FS, SO, cnt, lane);
 }
 
 // ---------------------------------------------------------------------------
 // kernel B: the transposed RHS
 // ---------------------------------------------------------------------------
+
+// Write the batch's transposed RHS OUT[e*ES + w] (LDS): to `out` (mode 0),
+// or the reverse LSERK4 update qbar += dt OUT, rbar *= a (mode 1).
+template <int ES>
+__device__ __forceinline__ unsigned adj_write_batch(const AdjArgs &a, const double *OUT, long e0,
+                                                    int cnt, int lane) {
+  constexpr int NV2 = 16 * ES / 2, NIT = (NV2 + 63) / 64;
+  const long base = e0 * ES;
+  double *rb = (double *)a.lam;
+  unsigned ef = 0;
+  if (cnt == 16) {
+    dbl2 qv[NIT], rv[NIT];
+    if (a.mode == 1) {
+#pragma unroll
+      for (int j = 0; j < NIT; ++j) {
+        const int idx = lane + 64 * j;
+        if (idx < NV2) {
+          qv[j] = ((const dbl2 *)(a.qbar + base))[idx];
+          rv[j] = ((const dbl2 *)(rb + base))[idx];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NIT; ++j) {
+      const int idx = lane + 64 * j;
+      if (idx < NV2) {
+        const dbl2 v = dbl2{OUT[2 * idx], OUT[2 * idx + 1]};
+        if (v[0] != v[0] || v[1] != v[1]) ef |= EF_NAN;
+        if (a.mode == 0) {
+          ((dbl2 *)(a.out + base))[idx] = v;
+        } else {
+          ((dbl2 *)(a.qbar + base))[idx] = qv[j] + a.dt * v;
+          ((dbl2 *)(rb + base))[idx] = a.rk_a * rv[j];
+        }
+      }
+    }
+  } else {
+    for (int i = lane; i < cnt * ES; i += 64) {
+      const double val = OUT[i];
+      if (val != val) ef |= EF_NAN;
+      if (a.mode == 0) {
+        a.out[base + i] = val;
+      } else {
+        a.qbar[base + i] += a.dt * val;
+        rb[base + i] *= a.rk_a;
+      }
+    }
+  }
+  return ef;
+}
 
 template <int N, int EQN>
 struct AdjLds {
@@ -290,11 +402,26 @@ struct AdjLds {
   static constexpr int XT = Order<N>::NKS1 * 4 * M * 16;
   static constexpr int PT = 3 * Nfp * M * 16;
   static constexpr int I1D = 3 * Ng * Nfp;
-  static constexpr int doubles = 2 * XT + 2 * PT + I1D + 2 * Nfp;
+  static constexpr int doubles = 2 * XT + PT + I1D + 2 * Nfp;
+};
+
+// Waves per SIMD requested from the register allocator: two where the
+// kernel fits 256 VGPRs without spilling (measured with
+// -Rpass-analysis=kernel-resource-usage), one otherwise.
+template <int N, int EQN>
+struct AdjWaves {
+  static constexpr int value =
+      (EQN == APL_EQN_MAXWELL_TM3 || (EQN == APL_EQN_EULER4 && N <= 4)) ? 2 : 1;
+};
+
+template <int N, int EQN>
+struct AdjGroup {
+  static constexpr int W = group_waves(AdjLds<N, EQN>::doubles, 4 * AdjWaves<N, EQN>::value);
 };
 
 template <int N, int EQN, int FLUX>
-__global__ void __launch_bounds__(64) adj_rhs_kernel(AdjArgs a) {
+__global__ void __launch_bounds__((64 * AdjGroup<N, EQN>::W), (AdjWaves<N, EQN>::value))
+    adj_rhs_kernel(AdjArgs a) {
   using O = Order<N>;
   using L = AdjLds<N, EQN>;
   constexpr int M = Vars<EQN>::M;
@@ -303,32 +430,32 @@ __global__ void __launch_bounds__(64) adj_rhs_kernel(AdjArgs a) {
   constexpr int ES = Np * M, FS = 3 * Ng * M;
   typedef typename Acc<double>::type V4;
 
-  extern __shared__ double lds[];
+  constexpr int W = AdjGroup<N, EQN>::W;
+  const WaveBatch wb = wave_batch<W>(a.K);
+  if (!wb.active) return;
+  extern __shared__ double lds_all[];
+  double *lds = lds_all + wb.wave * L::doubles;
   double *XT = lds;                 // state, transposed
   double *LT = XT + L::XT;          // lam / J, transposed
   double *PT = LT + L::XT;          // neighbour face nodes (own order)
-  double *QT = PT + L::PT;          // own face nodes
-  double *I1 = QT + L::PT;
+  double *I1 = PT + L::PT;
   int *FM = (int *)(I1 + L::I1D);
 
-  const int lane = threadIdx.x, el = lane & 15, lg = lane >> 4;
-  const long e0 = (long)xcd_batch(blockIdx.x, gridDim.x) * 16;
-  const int cnt = (int)min((long)16, (long)a.K - e0);
+  const int lane = wb.lane, el = lane & 15, lg = lane >> 4;
+  const long e0 = wb.e0;
+  const int cnt = wb.cnt;
   const long ge = e0 + (el < cnt ? el : 0);
   const double *gk = a.geom + ge * G_WORDS;
   const double jrx = gk[G_JRX], jry = gk[G_JRY], jsx = gk[G_JSX], jsy = gk[G_JSY];
 
+  // 1/J of lane el's element (0 past the batch end) scales the lam operands
+  const double sc = el < cnt ? gk[G_INVJ] : 0.0;
   for (int i = lane; i < L::I1D; i += 64) I1[i] = a.I1D[i];
   for (int i = lane; i < 3 * Nfp; i += 64) FM[i] = a.fmask[i];
-  const double *lam = a.lam;
-  for (int i = lane; i < 16 * ES; i += 64) {
-    const int e = i / ES, w = i - e * ES;
-    const int es = e < cnt ? e : 0;
-    XT[w * 16 + e] = a.q[(e0 + es) * ES + w];
-    LT[w * 16 + e] = (e < cnt) ? lam[(e0 + e) * ES + w] / a.geom[(e0 + e) * G_WORDS + G_J] : 0.0;
-  }
+  stage_transposed<ES>(XT, a.q + e0 * ES, cnt, lane);
+  stage_transposed<ES>(LT, a.lam + e0 * ES, cnt, lane);
   for (int i = 16 * ES + lane; i < L::XT; i += 64) XT[i] = LT[i] = 0.0;
-  __syncthreads();
+  wave_sync();
   for (int it = lane; it < 16 * 3 * Nfp; it += 64) {
     const int e = it & 15, k = it >> 4, f = k / Nfp, i = k - f * Nfp;
     const long gee = e0 + (e < cnt ? e : 0);
@@ -339,10 +466,9 @@ __global__ void __launch_bounds__(64) adj_rhs_kernel(AdjArgs a) {
       if (code >= 0)
         pv = a.q[((long)(code >> 2) * Np + FM[(code & 3) * Nfp + (Nfp - 1 - i)]) * M + v];
       PT[(k * M + v) * 16 + e] = pv;
-      QT[(k * M + v) * 16 + e] = XT[(FM[f * Nfp + i] * M + v) * 16 + e];
     }
   }
-  __syncthreads();
+  wave_sync();
 
   V4 acc[NRT][M];
 #pragma unroll
@@ -364,7 +490,7 @@ __global__ void __launch_bounds__(64) adj_rhs_kernel(AdjArgs a) {
 #pragma unroll
       for (int v = 0; v < M; ++v) {
         const double bx = XT[((4 * ks + lg) * M + v) * 16 + el];
-        const double bl = LT[((4 * ks + lg) * M + v) * 16 + el];
+        const double bl = LT[((4 * ks + lg) * M + v) * 16 + el] * sc;
         qc[v] = mma(a1, bx, qc[v]);
         fr[v] = mma(ar, bl, fr[v]);
         fs[v] = mma(as, bl, fs[v]);
@@ -420,9 +546,11 @@ __global__ void __launch_bounds__(64) adj_rhs_kernel(AdjArgs a) {
       for (int v = 0; v < M; ++v) qM[v] = qP[v] = qPb[v] = 0.0;
       const double *Io = I1 + (f * Ng + g) * Nfp;
 #pragma unroll
-      for (int i = 0; i < Nfp; ++i)
+      for (int i = 0; i < Nfp; ++i) {
+        const double *xo = XT + FM[f * Nfp + i] * M * 16 + el;
 #pragma unroll
-        for (int v = 0; v < M; ++v) qM[v] += Io[i] * QT[((f * Nfp + i) * M + v) * 16 + el];
+        for (int v = 0; v < M; ++v) qM[v] += Io[i] * xo[v * 16];
+      }
       const double *so = a.sbar + ge * FS + kf * M;
 #pragma unroll
       for (int v = 0; v < M; ++v) nb[v] = so[v] * sj;
@@ -467,7 +595,7 @@ __global__ void __launch_bounds__(64) adj_rhs_kernel(AdjArgs a) {
   }
 
   // epilogue
-  __syncthreads();
+  wave_sync();
   double *OUT = lds;
 #pragma unroll
   for (int rt = 0; rt < NRT; ++rt)
@@ -479,21 +607,10 @@ __global__ void __launch_bounds__(64) adj_rhs_kernel(AdjArgs a) {
         for (int v = 0; v < M; ++v) OUT[el * ES + n * M + v] = acc[rt][v][r];
       }
     }
-  __syncthreads();
-  const long base = e0 * ES;
-  for (int i = lane; i < cnt * ES; i += 64) {
-    const double val = OUT[i];
-    if (val != val) ef |= EF_NAN;
-    if (a.mode == 0) {
-      a.out[base + i] = val;
-    } else {
-      a.qbar[base + i] += a.dt * val;
-      double *rb = (double *)a.lam;
-      rb[base + i] *= a.rk_a;
-    }
-  }
+  wave_sync();
+  if (wb.active) ef |= adj_write_batch<ES>(a, OUT, e0, cnt, lane);
   for (int off = 32; off > 0; off >>= 1) ef |= __shfl_xor(ef, off);
-  if (lane == 0 && ef) atomicOr(a.err, ef);
+  if (lane == 0 && ef && wb.active) atomicOr(a.err, ef);
 }
 
 // ---------------------------------------------------------------------------

@@ -338,6 +338,70 @@ __device__ __forceinline__ int xcd_batch(int b, int nb) {
   return x * q + (x < r ? x : r) + i;
 }
 
+// Waves per workgroup.  Each wave owns one 16-element batch and its own LDS
+// slice; workgroups of 4 such waves stream HBM about twice as fast as
+// single-wave workgroups (measured on the fused-stage traffic pattern).  A
+// group size W keeps eff(W) = min(waves whose LDS fits a CU's 160 KB,
+// target) waves resident, where target is what the registers allow; the
+// largest W with eff(W) >= 3/4 of the best eff is taken.
+__host__ __device__ constexpr int lds_resident_waves(int w, int lds_bytes_per_wave) {
+  return w * lds_bytes_per_wave > 163840 ? 0 : (163840 / (w * lds_bytes_per_wave)) * w;
+}
+
+__host__ __device__ constexpr int eff_waves(int w, int lds_doubles_per_wave, int target) {
+  return lds_resident_waves(w, 8 * lds_doubles_per_wave) < target
+             ? lds_resident_waves(w, 8 * lds_doubles_per_wave)
+             : target;
+}
+
+__host__ __device__ constexpr int group_waves(int lds_doubles_per_wave, int target_waves_per_cu) {
+  // best of the three sizes, then the largest size within 3/4 of it
+  return 4 * eff_waves(4, lds_doubles_per_wave, target_waves_per_cu) >=
+                 3 * (eff_waves(2, lds_doubles_per_wave, target_waves_per_cu) >
+                              eff_waves(1, lds_doubles_per_wave, target_waves_per_cu)
+                          ? eff_waves(2, lds_doubles_per_wave, target_waves_per_cu)
+                          : eff_waves(1, lds_doubles_per_wave, target_waves_per_cu)) &&
+                 eff_waves(4, lds_doubles_per_wave, target_waves_per_cu) > 0
+             ? 4
+             : (4 * eff_waves(2, lds_doubles_per_wave, target_waves_per_cu) >=
+                        3 * eff_waves(1, lds_doubles_per_wave, target_waves_per_cu) &&
+                    eff_waves(2, lds_doubles_per_wave, target_waves_per_cu) > 0
+                    ? 2
+                    : 1);
+}
+
+// Orders one wave's LDS accesses across its lanes.  A wave only touches its
+// own LDS slice, so no workgroup barrier is needed: a workgroup-wide barrier
+// would make the group's waves wait for each other's gathers.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The batch of this wave: workgroup g (XCD-remapped) holds batches g W ..
+// g W + W - 1.  A wave past the last batch returns at once.
+struct WaveBatch {
+  int lane, wave, b, cnt;
+  long e0;
+  bool active;
+};
+
+template <int W>
+__device__ __forceinline__ WaveBatch wave_batch(int K) {
+  WaveBatch w;
+  // lane id from mbcnt: the compiler knows its range [0, 64)
+  w.lane = W == 1 ? (int)threadIdx.x : (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+  w.wave = W == 1 ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform in the wave
+  const int nb = (K + 15) / 16;
+  const int bq = xcd_batch(blockIdx.x, gridDim.x) * W + w.wave;
+  w.active = bq < nb;
+  w.b = w.active ? bq : nb - 1;
+  w.e0 = (long)w.b * 16;
+  w.cnt = (int)min((long)16, (long)K - w.e0);
+  return w;
+}
+
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 
 // XT[w*16 + e] = q[(e0 + e)*ES + w]: all loads of a full batch are issued
@@ -553,8 +617,17 @@ struct MinWaves {
       (EQN == APL_EQN_PHMAXWELL8 || N >= 6) ? 1 : (N == 5 ? (EQN == APL_EQN_MAXWELL_TM3 ? 2 : 1) : 2);
 };
 
+// Split-BF16x3 cubature kernels stay at one wave per group: with the barriers
+// of a multi-wave group their schedule no longer fits 256 VGPRs (measured).
+template <int N, int EQN, typename T>
+struct RhsGroup {
+  static constexpr int W =
+      sizeof(T) == 1 ? 1 : group_waves(Lds<N, EQN, T>::doubles, 4 * MinWaves<N, EQN>::value);
+};
+
 template <int N, int EQN, int FLUX, typename T>
-__global__ void __launch_bounds__(64, (MinWaves<N, EQN>::value)) rhs_kernel(RhsArgs a) {
+__global__ void __launch_bounds__((64 * RhsGroup<N, EQN, T>::W), (MinWaves<N, EQN>::value))
+    rhs_kernel(RhsArgs a) {
   using O = Order<N>;
   using L = Lds<N, EQN, T>;
   constexpr int M = Vars<EQN>::M;
@@ -568,16 +641,20 @@ __global__ void __launch_bounds__(64, (MinWaves<N, EQN>::value)) rhs_kernel(RhsA
   // k index 8 (lane/16) + item stands for the FP32 k-step 8 kk + item
   constexpr int NK8 = (NKS1 + 7) / 8, NKF8 = (NKF + 7) / 8;
 
-  extern __shared__ double lds[];
+  constexpr int W = RhsGroup<N, EQN, T>::W;
+  const WaveBatch wb = wave_batch<W>(a.K);
+  if (!wb.active) return;
+  extern __shared__ double lds_all[];
+  double *lds = lds_all + wb.wave * L::doubles;
   double *XT = lds;
   double *PT = lds + L::XT;
   double *I1 = PT + L::PT;
   int *FM = (int *)(I1 + L::I1D);
 
-  const int lane = threadIdx.x;
+  const int lane = wb.lane;
   const int el = lane & 15, lg = lane >> 4;
-  const long e0 = (long)xcd_batch(blockIdx.x, gridDim.x) * 16;
-  const int cnt = (int)min((long)16, (long)a.K - e0);
+  const long e0 = wb.e0;
+  const int cnt = wb.cnt;
 
   // element data of this lane (element el; missing elements repeat element 0)
   const long ge = e0 + (el < cnt ? el : 0);
@@ -596,7 +673,7 @@ __global__ void __launch_bounds__(64, (MinWaves<N, EQN>::value)) rhs_kernel(RhsA
   // 2. stage the batch state, transposed (coalesced global reads)
   stage_transposed<ES>(XT, a.q + e0 * ES, cnt, lane);
   for (int i = 16 * ES + lane; i < L::XT; i += 64) XT[i] = 0.0;
-  __syncthreads();
+  wave_sync();
 
   // 3. gather the neighbours' face nodes; item (k = f*Nfp + i, e), e fastest
   constexpr int NGI = (16 * 3 * Nfp + 63) / 64;
@@ -622,7 +699,7 @@ __global__ void __launch_bounds__(64, (MinWaves<N, EQN>::value)) rhs_kernel(RhsA
       for (int v = 0; v < M; ++v) PT[(k * M + v) * 16 + e] = v8[v];
     }
   }
-  __syncthreads();
+  wave_sync();
 
   V4 acc[NRT][M];
 #pragma unroll
@@ -803,7 +880,7 @@ __global__ void __launch_bounds__(64, (MinWaves<N, EQN>::value)) rhs_kernel(RhsA
   ef |= F.ef;
 
   // 6. epilogue: / J, stage through LDS as [e][n][v], coalesced write
-  __syncthreads();
+  wave_sync();
   double *OUT = lds;
 #pragma unroll
   for (int rt = 0; rt < NRT; ++rt)
@@ -815,16 +892,16 @@ __global__ void __launch_bounds__(64, (MinWaves<N, EQN>::value)) rhs_kernel(RhsA
         for (int v = 0; v < M; ++v) OUT[el * ES + n * M + v] = double(acc[rt][v][r]) / Jel;
       }
     }
-  __syncthreads();
-  ef |= write_batch<ES>(a, OUT, e0, cnt, lane);
+  wave_sync();
+  if (wb.active) ef |= write_batch<ES>(a, OUT, e0, cnt, lane);
   // wave-wide max speed and flags
   for (int off = 32; off > 0; off >>= 1) {
     const double o = __shfl_xor(lmax, off);
     lmax = o > lmax ? o : lmax;
     ef |= __shfl_xor(ef, off);
   }
-  if (lane == 0) {
-    if (a.speed_part) a.speed_part[blockIdx.x] = lmax;
+  if (lane == 0 && wb.active) {
+    if (a.speed_part) a.speed_part[wb.b] = lmax;
     if (ef) atomicOr(a.err, ef);
   }
 }
@@ -884,12 +961,22 @@ __device__ __forceinline__ void gather_face_node(const double *q, int kf, int co
 template <int N, int EQN, typename T>
 struct LinWaves {
   static constexpr int value = EQN == APL_EQN_PHMAXWELL8 ? 1
-                              : sizeof(T) == 8 ? (N <= 6 ? 2 : 1)
-                                               : (N <= 6 ? 4 : 2);   // FP32 and BF16x3
+                              : sizeof(T) == 8 ? (N <= 5 ? 2 : 1)
+                                               : (N <= 5 ? 4 : (N == 6 ? 3 : 2));   // FP32, BF16x3
+};
+
+// The nodal kernel keeps one wave per group: it is register-bound at the
+// same waves per CU either way, and a multi-wave group holds its LDS until
+// its slowest wave ends (measured at N=5: 65.5 us per stage with one wave,
+// 69.0 us with four).
+template <int N, int EQN, typename T>
+struct LinGroup {
+  static constexpr int W = 1;
 };
 
 template <int N, int EQN, int FLUX, typename T>
-__global__ void __launch_bounds__(64, (LinWaves<N, EQN, T>::value)) rhs_linear_kernel(RhsArgs a) {
+__global__ void __launch_bounds__((64 * LinGroup<N, EQN, T>::W), (LinWaves<N, EQN, T>::value))
+    rhs_linear_kernel(RhsArgs a) {
   using O = Order<N>;
   using L = LinLds<N, EQN, T>;
   constexpr int M = Vars<EQN>::M;
@@ -904,13 +991,17 @@ __global__ void __launch_bounds__(64, (LinWaves<N, EQN, T>::value)) rhs_linear_k
   // lane's 8 items are the values it holds for eight 4-row steps
   constexpr int NK8 = (NKS1 + 7) / 8, NKP8 = (NKP + 7) / 8;
 
-  extern __shared__ double lds[];
+  constexpr int W = LinGroup<N, EQN, T>::W;
+  const WaveBatch wb = wave_batch<W>(a.K);
+  if (!wb.active) return;
+  extern __shared__ double lds_all[];
+  double *lds = lds_all + wb.wave * L::doubles;
   double *XT = lds;
 
-  const int lane = threadIdx.x;
+  const int lane = wb.lane;
   const int el = lane & 15, lg = lane >> 4;
-  const long e0 = (long)xcd_batch(blockIdx.x, gridDim.x) * 16;
-  const int cnt = (int)min((long)16, (long)a.K - e0);
+  const long e0 = wb.e0;
+  const int cnt = wb.cnt;
 
   const long ge = e0 + (el < cnt ? el : 0);
   const double *gk = a.geom + ge * G_WORDS;
@@ -942,7 +1033,7 @@ __global__ void __launch_bounds__(64, (LinWaves<N, EQN, T>::value)) rhs_linear_k
 
   stage_transposed<ES>(XT, a.q + e0 * ES, cnt, lane);
   for (int i = 16 * ES + lane; i < L::XT; i += 64) XT[i] = 0.0;
-  __syncthreads();
+  wave_sync();
 
   unsigned ef = 0;
   const S *A1 = (const S *)a.A1;   // [2][NRT][NKS1][64]: Dt_r, Dt_s
@@ -1078,7 +1169,7 @@ __global__ void __launch_bounds__(64, (LinWaves<N, EQN, T>::value)) rhs_linear_k
 
   // epilogue: / J, stage through LDS as [e][n][v], coalesced write
   const double Jel = gk[G_J];
-  __syncthreads();
+  wave_sync();
   double *OUT = lds;
 #pragma unroll
   for (int rt = 0; rt < NRT; ++rt)
@@ -1090,15 +1181,15 @@ __global__ void __launch_bounds__(64, (LinWaves<N, EQN, T>::value)) rhs_linear_k
         for (int v = 0; v < M; ++v) OUT[el * ES + n * M + v] = double(acc[rt][v][r]) / Jel;
       }
     }
-  __syncthreads();
-  ef |= write_batch<ES>(a, OUT, e0, cnt, lane);
+  wave_sync();
+  if (wb.active) ef |= write_batch<ES>(a, OUT, e0, cnt, lane);
   for (int off = 32; off > 0; off >>= 1) {
     const double o = __shfl_xor(lmax, off);
     lmax = o > lmax ? o : lmax;
     ef |= __shfl_xor(ef, off);
   }
-  if (lane == 0) {
-    if (a.speed_part) a.speed_part[blockIdx.x] = lmax;
+  if (lane == 0 && wb.active) {
+    if (a.speed_part) a.speed_part[wb.b] = lmax;
     if (ef) atomicOr(a.err, ef);
   }
 }

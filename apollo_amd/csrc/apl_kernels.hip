@@ -10,30 +10,34 @@
 
 namespace apl {
 
+template <typename K>
+static bool allow_lds(K kern, size_t lds) {
+  return lds <= 65536 || hipFuncSetAttribute((const void *)kern,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)lds) == hipSuccess;
+}
+
+// Launch `kern` over `nbatches` 16-element batches in workgroups of W waves
+// (one batch per wave, LDS_DOUBLES doubles of LDS per wave).
+template <int LDS_DOUBLES, int W, typename K, typename A>
+static int launch_batches(K kern, hipStream_t s, const A &a, int nbatches) {
+  const size_t lds = sizeof(double) * (size_t)LDS_DOUBLES * W;
+  if (!allow_lds(kern, lds)) return APL_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL(kern, dim3((nbatches + W - 1) / W), dim3(64 * W), lds, s, a);
+  return hipGetLastError() == hipSuccess ? APL_OK : APL_ERR_HIP;
+}
+
 template <int N, int EQN, int FLUX, typename T>
 static int launch_typed(int linear, hipStream_t s, const RhsArgs &a, int nblocks) {
   if constexpr (EQN == APL_EQN_PHMAXWELL8 || EQN == APL_EQN_MAXWELL_TM3) {
-    if (linear) {
-      const size_t lds = sizeof(double) * LinLds<N, EQN, T>::doubles;
-      auto kern = rhs_linear_kernel<N, EQN, FLUX, T>;
-      if (lds > 65536 && hipFuncSetAttribute((const void *)kern,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                                             (int)lds) != hipSuccess)
-        return APL_ERR_UNSUPPORTED;
-      hipLaunchKernelGGL(kern, dim3(nblocks), dim3(64), lds, s, a);
-      return hipGetLastError() == hipSuccess ? APL_OK : APL_ERR_HIP;
-    }
+    if (linear)
+      return launch_batches<LinLds<N, EQN, T>::doubles, LinGroup<N, EQN, T>::W>(
+          rhs_linear_kernel<N, EQN, FLUX, T>, s, a,
+                                                         nblocks);
   }
   if (linear) return APL_ERR_ARG;
-  const size_t lds = sizeof(double) * Lds<N, EQN, T>::doubles;
-  auto kern = rhs_kernel<N, EQN, FLUX, T>;
-  if (lds > 65536) {
-    if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds) != hipSuccess)
-      return APL_ERR_UNSUPPORTED;
-  }
-  hipLaunchKernelGGL(kern, dim3(nblocks), dim3(64), lds, s, a);
-  return hipGetLastError() == hipSuccess ? APL_OK : APL_ERR_HIP;
+  return launch_batches<Lds<N, EQN, T>::doubles, RhsGroup<N, EQN, T>::W>(rhs_kernel<N, EQN, FLUX, T>, s,
+                                                                          a, nblocks);
 }
 
 template <int N, int EQN, int FLUX>
@@ -66,24 +70,13 @@ int launch_rhs(int eqn, int flux, int prec, int linear, hipStream_t s, const Rhs
 
 template int launch_rhs<APL_ORDER>(int, int, int, int, hipStream_t, const RhsArgs &, int);
 
-template <typename K>
-static bool allow_lds(K kern, size_t lds) {
-  return lds <= 65536 || hipFuncSetAttribute((const void *)kern,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                                             (int)lds) == hipSuccess;
-}
-
 template <int N, int EQN, int FLUX>
 static int launch_adjoint_typed(hipStream_t s, const AdjArgs &a, int nblocks) {
-  auto ka = adj_face_kernel<N, EQN>;
-  auto kb = adj_rhs_kernel<N, EQN, FLUX>;
-  const size_t la = sizeof(double) * AdjFaceLds<N, EQN>::doubles;
-  const size_t lb = sizeof(double) * AdjLds<N, EQN>::doubles;
-  if (!allow_lds(ka, la) || !allow_lds(kb, lb)) return APL_ERR_UNSUPPORTED;
-  hipLaunchKernelGGL(ka, dim3(nblocks), dim3(64), la, s, a);
-  if (hipGetLastError() != hipSuccess) return APL_ERR_HIP;
-  hipLaunchKernelGGL(kb, dim3(nblocks), dim3(64), lb, s, a);
-  return hipGetLastError() == hipSuccess ? APL_OK : APL_ERR_HIP;
+  const int rc = launch_batches<AdjFaceLds<N, EQN>::doubles, AdjFaceGroup<N, EQN>::W>(
+      adj_face_kernel<N, EQN>, s, a, nblocks);
+  if (rc) return rc;
+  return launch_batches<AdjLds<N, EQN>::doubles, AdjGroup<N, EQN>::W>(adj_rhs_kernel<N, EQN, FLUX>, s, a,
+                                                                       nblocks);
 }
 
 template <int N>

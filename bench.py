@@ -19,7 +19,10 @@ timed region starts.
 The adjoint leg (Euler isentropic vortex, [-5,5]^2 periodic, 256x256x2
 jittered 0.1h seed 3, N = 3, 200 steps, checkpoint every 20) times the
 reverse LSERK4 and checks dJ/dq0 against central finite differences along 8
-Gaussian directions (eps = 1e-6).
+Gaussian directions (eps = 1e-6).  Two more legs time FP64, FP32-MFMA and
+split-BF16x3 on the Euler vortex at N = 4 (512x512x2, K = 524288, 200 steps,
+density error against the FP64 run) and on Maxwell TM at N = 8 (384x384x2,
+K = 294912, random 16-mode Ez, 300 steps, MFMA utilisation).
 
     python bench.py [--gpus 1] [--steps 500] [--warmup 20] [--workload c2]
 """
@@ -226,6 +229,40 @@ def run_adjoint(torch, steps=200, ckpt=20, ndir=8, eps=1e-6):
     return res
 
 
+def run_leg(name, steps, variants, torch):
+    """A secondary workload timed like the headline one: DOF-updates/s and
+    the L2 error of each variant against the FP64 run after `steps` steps
+    (density only for Euler)."""
+    w = workload(name)
+    res = {}
+    for vname, prec, algo in variants:
+        print("bench: %s %s steps=%d" % (name, vname, steps), file=sys.stderr, flush=True)
+        res[vname] = run_gpu(w, prec, steps, 5, torch, algo)
+    info = res["fp64"]["info"]
+    K, N, Np, Nc, Ng, m = (info[k] for k in ("K", "N", "Np", "Nc", "Ng", "m"))
+    ref = res["fp64"]["qT"]
+    sel = (lambda q: q[..., 0]) if m == 4 else (lambda q: q)
+    out = {"workload": w["desc"], "steps": steps, "dt": info["dt"], "variants": {}}
+    for vname, prec, algo in variants:
+        r = res[vname]
+        kern_s = r["gpu_ms"] * 1e-3 / (5 * steps)
+        lin = m == 3 and algo == 0
+        d = dict(dof_updates_per_s=K * Np * m / kern_s, kernel_us=kern_s * 1e6,
+                 form="nodal" if lin else "cubature",
+                 tflops_exec=flop_exec(lin, K, N, Np, Nc, Ng, m) / kern_s / 1e12)
+        if vname != "fp64":
+            d["l2_vs_fp64"] = float(np.linalg.norm((sel(r["qT"]) - sel(ref)).ravel())
+                                    / np.linalg.norm(sel(ref).ravel()))
+        ex = exact_state(w, r["x"], r["y"], steps * r["dt"])
+        if ex is not None:
+            d["l2_vs_analytic"] = float(np.linalg.norm((sel(r["qT"]) - sel(ex)).ravel())
+                                        / np.linalg.norm(sel(ex).ravel()))
+        pk = PEAK["fp64" if vname == "fp64" else ("fp32" if vname == "fp32" else "bf16")]
+        d["mfma_frac_exec"] = d["tflops_exec"] * 1e12 / pk
+        out["variants"][vname] = d
+    return out
+
+
 def cpu_baseline(w, nthreads, budget_s=15.0):
     """Oracle RHS (Apollo's loop restated in C, Apollo's 8-variable Maxwell /
     5-variable Euler layout) on the host cores, on the full mesh."""
@@ -292,6 +329,8 @@ def main():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="skip the Euler N=4 and Maxwell N=8 workloads")
     ap.add_argument("--only", default="",
                     help="time one variant only (fp64, fp32, bf16x3, fp64_cub, fp32_cub, bf16x3_cub)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -405,6 +444,11 @@ def main():
     if args.workload == "c2" and not args.no_adjoint and not args.only:
         print("bench: adjoint (c4)", file=sys.stderr, flush=True)
         line["adjoint"] = run_adjoint(torch)
+    if args.workload == "c2" and not args.no_legs and not args.only:
+        three = [("fp64", E.PREC_FP64, E.ALGO_AUTO), ("fp32", E.PREC_FP32_MFMA, E.ALGO_AUTO),
+                 ("bf16x3", E.PREC_BF16X3, E.ALGO_AUTO)]
+        line["euler_c3"] = run_leg("c3", 200, three, torch)
+        line["maxwell_c5"] = run_leg("c5", 300, three, torch)
     if not args.no_cpu_baseline:
         nth = min(16, os.cpu_count() or 1)
         print("bench: cpu baseline, %d threads" % nth, file=sys.stderr, flush=True)
