@@ -87,7 +87,8 @@ struct BCTable {
 
 struct RhsArgs {
   int K;
-  int mode;                // 0: write rhs to `out`; 1: LSERK4 stage update
+  int mode;                // 0: write rhs to `out`; 1: LSERK4 stage update;
+                           // 2: out = ca q + cb rhs (+ cz z)
   const double *q;         // [K][Np][M] stage input
   double *out;             // rhs (mode 0) or the stage output state (mode 1)
   double *res;             // LSERK4 residual (mode 1)
@@ -102,6 +103,8 @@ struct RhsArgs {
   const double *tptr;      // device time (graph replay) or null
   double t, tofs;          // time = (tptr ? *tptr : t) + tofs
   double rk_a, rk_b, dt;   // stage coefficients (mode 1)
+  const double *z;         // mode 2: third vector (own element only), or null
+  double ca, cb, cz;       // mode 2 coefficients
   double *speed_part;      // [nbatch] max LF speed per batch, or null
   unsigned *err;           // error flags
   PhysParams ph;
@@ -437,8 +440,10 @@ __device__ __forceinline__ void stage_transposed(double *XT, const double *qb, i
   }
 }
 
-// Write the batch result OUT[e*ES + w] (LDS): rhs (mode 0) or the LSERK4
-// stage update res = a res + dt rhs, q_out = q_in + b res (mode 1).
+// Write the batch result OUT[e*ES + w] (LDS): rhs (mode 0), the LSERK4
+// stage update res = a res + dt rhs, q_out = q_in + b res (mode 1), or the
+// linear combination out = ca q + cb rhs (+ cz z) of an SSP stage (mode 2;
+// z may be `out` itself: each element reads its own z before writing).
 template <int ES>
 __device__ __forceinline__ unsigned write_batch(const RhsArgs &a, const double *OUT, long e0,
                                                 int cnt, int lane) {
@@ -446,6 +451,39 @@ __device__ __forceinline__ unsigned write_batch(const RhsArgs &a, const double *
   constexpr int NIT = (NV2 + 63) / 64;
   const long base = e0 * ES;
   unsigned ef = 0;
+  if (a.mode == 2) {
+    if (cnt == 16) {
+      dbl2 qv[NIT], zv[NIT];
+#pragma unroll
+      for (int j = 0; j < NIT; ++j) {
+        const int idx = lane + 64 * j;
+        if (idx < NV2) {
+          qv[j] = ((const dbl2 *)(a.q + base))[idx];
+          if (a.z) zv[j] = ((const dbl2 *)(a.z + base))[idx];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < NIT; ++j) {
+        const int idx = lane + 64 * j;
+        if (idx < NV2) {
+          const dbl2 v = dbl2{OUT[2 * idx], OUT[2 * idx + 1]};
+          if (v[0] != v[0] || v[1] != v[1]) ef |= EF_NAN;
+          dbl2 o = a.ca * qv[j] + a.cb * v;
+          if (a.z) o = o + a.cz * zv[j];
+          ((dbl2 *)(a.out + base))[idx] = o;
+        }
+      }
+    } else {
+      for (int i = lane; i < cnt * ES; i += 64) {
+        const double val = OUT[i];
+        if (val != val) ef |= EF_NAN;
+        double o = a.ca * a.q[base + i] + a.cb * val;
+        if (a.z) o = o + a.cz * a.z[base + i];
+        a.out[base + i] = o;
+      }
+    }
+    return ef;
+  }
   if (cnt == 16) {
     if (a.mode == 0) {
 #pragma unroll
@@ -1196,6 +1234,9 @@ __global__ void __launch_bounds__((64 * LinGroup<N, EQN, T>::W), (LinWaves<N, EQ
 
 // Advance the device clock by dt at the end of a captured step.
 int launch_advance_time(hipStream_t s, double *t, double dt);
+
+// out[0] = max(in[0 .. n)), one workgroup.
+int launch_max_reduce(hipStream_t s, const double *in, int n, double *out);
 
 // Launch entry of one order (defined in apl_kernels_N<N>.hip); `linear`
 // selects the nodal kernel (Maxwell only).

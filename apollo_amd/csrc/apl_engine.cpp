@@ -844,6 +844,89 @@ int apl_lserk4(apl_handle *h, double t0, double dt, int nsteps, double *q, int w
 }
 
 // ---------------------------------------------------------------------------
+// Apollo's stepper within one output frame: PETSc TSSSP (type rks2, s stages)
+// under ApSolver's dt control
+// ---------------------------------------------------------------------------
+
+// One frame as ApSolver::solve runs it (src/solvers/apsolver.cc:172-216):
+// TSSetInitialTimeStep(t0, dt0), exact final time MATCHSTEP, and per step
+//  - MonitorVTK (apsolver.cc:421-440): dt = |t_end - t| if that is smaller;
+//  - MATCHSTEP: dt = t_end - t if t + dt would pass t_end;
+//  - TSSSP rks2 with s stages, the low-storage second-order SSP scheme
+//    (Ketcheson 2008): w = u; for i < s-1: w += dt/(s-1) L(t + dt i/(s-1), w);
+//    u = (s-1)/s w + dt/s L(t + dt, w) + u/s;
+//  - ComputeRHSforTS (apsolver.cc:442-480): after every RHS the next dt is
+//    min(suggested, dt of this step), so the last stage's value stands.
+// The stages run as fused kernels (epilogue mode 2); the last stage's speed
+// maximum is reduced on the device and read back once per step.
+int apl_ssp_advance(apl_handle *h, double t0, double t_end, double dt0, double cfl, int stages,
+                    double *q, int where, int max_steps, int *nsteps, double *dts) {
+  if (!h || !q || !(dt0 > 0.0) || stages < 2 || max_steps < 0 || !(t_end >= t0))
+    return APL_ERR_ARG;
+  int rc;
+  if ((rc = ensure_work(h))) return rc;
+  const size_t bytes = sizeof(double) * state_len(h);
+  double *sol = q;
+  if (where == APL_HOST) {
+    HIPCHK(h, hipMemcpyAsync(h->d_qa, q, bytes, hipMemcpyHostToDevice, h->stream));
+    sol = h->d_qa;
+  }
+  double *bufs[2] = {h->d_qb, h->d_res};
+  const int nb = (h->K + 15) / 16;
+  double *d_smax = h->d_t + 1;
+  HIPCHK(h, hipMemsetAsync(h->d_err, 0, sizeof(unsigned), h->stream));
+  RhsArgs a = base_args(h);
+  a.mode = 2;
+  double t = t0, dt_temp = dt0;
+  int n = 0;
+  for (;;) {
+    if (std::fabs(t_end - t) < dt_temp) dt_temp = std::fabs(t_end - t);   // MonitorVTK
+    if (t >= t_end) break;
+    double dt = dt_temp;
+    if (t + dt > t_end) dt = t_end - t;                                      // MATCHSTEP
+    if (n >= max_steps) return fail(h, APL_ERR_ARG, "apl_ssp_advance: more than max_steps steps");
+    const double *src = sol;
+    for (int i = 0; i < stages - 1; ++i) {
+      a.q = src;
+      a.out = bufs[i & 1];
+      a.z = nullptr;
+      a.ca = 1.0;
+      a.cb = dt / (stages - 1.);
+      a.cz = 0.0;
+      a.t = t + dt * (i / (stages - 1.));
+      a.speed_part = nullptr;
+      if ((rc = dispatch_rhs(h, a))) return rc;
+      src = bufs[i & 1];
+    }
+    a.q = src;
+    a.out = sol;
+    a.z = sol;
+    a.ca = (stages - 1.) / stages;
+    a.cb = dt / stages;
+    a.cz = 1. / stages;
+    a.t = t + dt;
+    a.speed_part = h->d_speed;
+    if ((rc = dispatch_rhs(h, a))) return rc;
+    if (launch_max_reduce(h->stream, h->d_speed, nb, d_smax)) return fail(h, APL_ERR_HIP, "max kernel");
+    double ms = 0.0;
+    unsigned ef = 0;
+    HIPCHK(h, hipMemcpyAsync(&ms, d_smax, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(&ef, h->d_err, sizeof(unsigned), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (ef) return check_flags(h);
+    // wxnodaldg2dmethod.cc:523, then apsolver.cc:474
+    const double suggested = 2. / 3. * cfl * h->geo.dtscale * (h->ops.rmin / ms);
+    dt_temp = std::fmin(suggested, dt);
+    t = t + dt;
+    if (dts) dts[n] = dt;
+    ++n;
+  }
+  if (nsteps) *nsteps = n;
+  if (where == APL_HOST) HIPCHK(h, hipMemcpy(q, sol, bytes, hipMemcpyDeviceToHost));
+  return check_flags(h);
+}
+
+// ---------------------------------------------------------------------------
 // discrete adjoint
 // ---------------------------------------------------------------------------
 

@@ -118,6 +118,28 @@ int launch_advance_time(hipStream_t s, double *t, double dt) {
   hipLaunchKernelGGL(advance_time_kernel, dim3(1), dim3(64), 0, s, t, dt);
   return hipGetLastError() == hipSuccess ? APL_OK : APL_ERR_HIP;
 }
+
+// One 256-lane workgroup: strided maxima, then a tree over LDS.
+__global__ void __launch_bounds__(256) max_reduce_kernel(const double *in, int n, double *out) {
+  __shared__ double part[256];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) m = in[i] > m ? in[i] : m;
+  part[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const double o = part[threadIdx.x + w];
+      part[threadIdx.x] = o > part[threadIdx.x] ? o : part[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = part[0];
+}
+
+int launch_max_reduce(hipStream_t s, const double *in, int n, double *out) {
+  hipLaunchKernelGGL(max_reduce_kernel, dim3(1), dim3(256), 0, s, in, n, out);
+  return hipGetLastError() == hipSuccess ? APL_OK : APL_ERR_HIP;
+}
 #endif
 
 }  // namespace apl

@@ -65,7 +65,8 @@ class _Info(ctypes.Structure):
 EXPORTS = [
     "apl_create", "apl_destroy", "apl_get_info", "apl_set_stream", "apl_get_index_maps",
     "apl_get_geometry", "apl_get_operator", "apl_rhs", "apl_initial_dt", "apl_lserk4",
-    "apl_adjoint_rhs", "apl_lserk4_adjoint", "apl_objective_disc_sq", "apl_mesh_structured",
+    "apl_ssp_advance", "apl_adjoint_rhs", "apl_lserk4_adjoint", "apl_objective_disc_sq",
+    "apl_mesh_structured",
     "apl_status_string", "apl_last_error", "apl_host_connectivity", "apl_host_operator",
 ]
 
@@ -99,6 +100,9 @@ def lib():
     L.apl_rhs.argtypes = [vp, ctypes.c_double, vp, vp, ctypes.c_double, _dp, ctypes.c_int]
     L.apl_initial_dt.argtypes = [vp, vp, ctypes.c_double, _dp, ctypes.c_int]
     L.apl_lserk4.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, ctypes.c_int]
+    L.apl_ssp_advance.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_int), _dp]
     L.apl_adjoint_rhs.argtypes = [vp, ctypes.c_double, vp, vp, vp, ctypes.c_int]
     L.apl_lserk4_adjoint.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                      ctypes.c_int, vp, vp, vp, vp, ctypes.c_int]
@@ -319,6 +323,16 @@ class Engine:
     def lserk4(self, t0, dt, nsteps, q):
         qp, where = self._ptr(q)
         self._check(lib().apl_lserk4(self._h, t0, dt, nsteps, qp, where), "lserk4")
+
+    def ssp_advance(self, t0, t_end, dt0, cfl, q, stages=5, max_steps=1000000):
+        """Apollo's stepping of one output frame (TSSSP rks2 under ApSolver's
+        dt control), in place.  Returns the per-step dt values."""
+        qp, where = self._ptr(q)
+        dts = np.zeros(max(max_steps, 1))
+        n = ctypes.c_int(0)
+        self._check(lib().apl_ssp_advance(self._h, t0, t_end, dt0, cfl, stages, qp, where,
+                                          max_steps, ctypes.byref(n), _d(dts)), "ssp_advance")
+        return dts[:n.value].copy()
 
     def adjoint_rhs(self, t, q, lam, out):
         qp, where = self._ptr(q)

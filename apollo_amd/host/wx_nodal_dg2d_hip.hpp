@@ -144,6 +144,29 @@ class WxNodalDG2dMethodHip {
     check(apl_lserk4(h_, t0, dt, nsteps, q, where), "advance");
   }
 
+  // One output frame [t0, t_end] as ApSolver::solve runs it
+  // (apsolver.cc:172-216): TSSSP rks2 with `stages` stages from dt0, the
+  // MonitorVTK clamp and ComputeRHSforTS's dt = min(suggested, dt).
+  // Returns the number of steps taken.
+  int advanceFrame(double t0, double t_end, double dt0, double *q, int stages = 5,
+                   int where = APL_HOST) {
+    int n = 0;
+    check(apl_ssp_advance(h_, t0, t_end, dt0, cfg_.cfl, stages, q, where, 100000000, &n, nullptr),
+          "advanceFrame");
+    return n;
+  }
+
+  // ApSolver::solve (apsolver.cc:172-216): nout frames of length t_end/nout,
+  // each restarted from dt, with `frame(i, t, q)` called after each frame.
+  void solve(double t_end, int nout, double dt, double *q,
+             const std::function<void(int, double, const double *)> &frame) {
+    const double tsize = t_end / nout;
+    for (int i = 0; i < nout; ++i) {
+      advanceFrame(i * tsize, (i + 1) * tsize, dt, q);
+      if (frame) frame(i, (i + 1) * tsize, q);
+    }
+  }
+
   unsigned totalEqns() const { return (unsigned)info_.m; }
   unsigned NpElem() const { return (unsigned)info_.Np; }
   unsigned numElements() const { return (unsigned)info_.K; }

@@ -197,6 +197,19 @@ int apl_initial_dt(apl_handle *h, const double *q, double cfl, double *dt, int w
  * (src/solvers/apsolver.cc:442-480, src/subsolvers/wxpetsctimestepping.h). */
 int apl_lserk4(apl_handle *h, double t0, double dt, int nsteps, double *q, int where);
 
+/* Apollo's own stepping of one output frame [t0, t_end], in place of the
+ * PETSc TSSolve that ApSolver::solve runs per frame (src/solvers/
+ * apsolver.cc:172-216): TSSSP (src/subsolvers/wxpetsctimestepping.h:33) of
+ * type rks2 with `stages` stages (PETSc's default is 5), TSSetInitialTimeStep
+ * (t0, dt0), exact final time MATCHSTEP, the MonitorVTK clamp to the frame
+ * end (apsolver.cc:421-440) and ComputeRHSforTS's dt = min(suggested, dt)
+ * after each RHS (apsolver.cc:442-480), with the suggested dt of
+ * WxNodalDG2dMethod::step at `cfl`.  q is advanced in place; *nsteps
+ * receives the step count and dts[0 .. *nsteps) (optional, max_steps long)
+ * each step's dt.  More than max_steps steps is APL_ERR_ARG. */
+int apl_ssp_advance(apl_handle *h, double t0, double t_end, double dt0, double cfl, int stages,
+                    double *q, int where, int max_steps, int *nsteps, double *dts);
+
 /* ---- discrete adjoint (no counterpart in the reference) ----------------- */
 
 /* out = (d rhs / d q)^T lam at state q and time t (FP64). */

@@ -827,6 +827,53 @@ int orc_lserk4(const orc_ops *o, const orc_phys *ph, int K, const double *x,
   return rc;
 }
 
+/* One output frame as ApSolver::solve advances it (src/solvers/apsolver.cc:
+ * 172-216): PETSc TSSSP (src/subsolvers/wxpetsctimestepping.h:33) of type
+ * rks2, the s-stage low-storage second-order SSP method (Ketcheson 2008;
+ * PETSc's default type, 5 stages), started by TSSetInitialTimeStep(t0, dt0)
+ * with exact final time MATCHSTEP (wxpetsctimestepping.h:92-96).  Per step:
+ * MonitorVTK clamps dt to the frame end (apsolver.cc:421-440), MATCHSTEP
+ * shortens a step that would pass t_end, and ComputeRHSforTS sets the next
+ * dt to min(suggested, this step's dt) after every RHS (apsolver.cc:
+ * 442-480), so the last stage's value stands.  The suggested dt is
+ * (2/3) cfl dtscale rmin / maxSpeed (wxnodaldg2dmethod.cc:523). */
+int orc_ssp_advance(const orc_ops *o, const orc_phys *ph, int K, const double *x,
+                    const double *y, const int *etetf, double t0, double t_end, double dt0,
+                    double cfl, double dtscale, int stages, double *q, int max_steps,
+                    int *nsteps, double *dts, int nthreads) {
+  size_t n = (size_t)K * o->Np * orc_meqn(ph);
+  double *w = ALLOC(double, n), *F = ALLOC(double, n);
+  double t = t0, dt_temp = dt0;
+  int rc = 0, steps = 0;
+  for (;;) {
+    if (fabs(t_end - t) < dt_temp) dt_temp = fabs(t_end - t);
+    if (t >= t_end) break;
+    double dt = dt_temp;
+    if (t + dt > t_end) dt = t_end - t;
+    if (steps >= max_steps) { rc = 1; break; }
+    double ms = 0.0;
+    memcpy(w, q, n * sizeof(double));
+    for (int i = 0; i < stages - 1 && !rc; i++) {
+      rc = orc_rhs(o, ph, K, x, y, etetf, t + dt * (i / (stages - 1.)), w, F, &ms, nthreads);
+      for (size_t k = 0; k < n; k++) w[k] = w[k] + dt / (stages - 1.) * F[k];
+    }
+    if (rc) break;
+    rc = orc_rhs(o, ph, K, x, y, etetf, t + dt, w, F, &ms, nthreads);
+    if (rc) break;
+    for (size_t k = 0; k < n; k++)
+      q[k] = (stages - 1.) / stages * w[k] + dt / stages * F[k] + 1. / stages * q[k];
+    double suggested = 2. / 3. * cfl * dtscale * (o->rmin / ms);
+    dt_temp = fmin(suggested, dt);
+    t = t + dt;
+    if (dts) dts[steps] = dt;
+    steps++;
+  }
+  if (nsteps) *nsteps = steps;
+  free(w);
+  free(F);
+  return rc;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Discrete adjoint (build extension)                                        */
 /* ------------------------------------------------------------------------ */

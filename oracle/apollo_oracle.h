@@ -128,6 +128,17 @@ int orc_adjoint_rhs(const orc_ops *ops, const orc_phys *ph, int K,
 
 /* Reverse LSERK4 with full recomputation from q0 (no checkpoint tricks):
  * dJdq0 = (d q_T / d q0)^T qbarT. */
+/* One output frame of Apollo's stepping: PETSc TSSSP rks2 (s stages) under
+ * ApSolver's dt control (apsolver.cc:172-216, 421-480).  Returns 0, an RHS
+ * error code, or 1 when more than max_steps steps would be needed.
+ * parity unpinned against PETSc (its TSSSP source is not in the reference
+ * tree): pinned by the method's stability polynomial, second-order
+ * convergence and the dt-control invariants in the tests. */
+int orc_ssp_advance(const orc_ops *ops, const orc_phys *ph, int K, const double *x,
+                    const double *y, const int *etetf, double t0, double t_end, double dt0,
+                    double cfl, double dtscale, int stages, double *q, int max_steps,
+                    int *nsteps, double *dts, int nthreads);
+
 int orc_lserk4_adjoint(const orc_ops *ops, const orc_phys *ph, int K,
                        const double *x, const double *y, const int *etetf,
                        double t0, double dt, int nsteps, const double *q0,

@@ -73,6 +73,11 @@ def lib():
                                  ctypes.c_int, _dp, ctypes.c_int]
         L.orc_adjoint_rhs.argtypes = [ctypes.POINTER(_Ops), ctypes.POINTER(_Phys), ctypes.c_int,
                                       _dp, _dp, _ip, ctypes.c_double, _dp, _dp, _dp]
+        L.orc_ssp_advance.argtypes = [ctypes.POINTER(_Ops), ctypes.POINTER(_Phys), ctypes.c_int,
+                                      _dp, _dp, _ip, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_int, _dp, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_int), _dp, ctypes.c_int]
         L.orc_lserk4_adjoint.argtypes = [ctypes.POINTER(_Ops), ctypes.POINTER(_Phys),
                                          ctypes.c_int, _dp, _dp, _ip, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_int, _dp, _dp, _dp]
@@ -282,6 +287,20 @@ def lserk4(mesh, ph, t0, dt, nsteps, q, nthreads=0):
     if rc:
         raise ValueError("orc_lserk4: %d" % rc)
     return q
+
+
+def ssp_advance(mesh, ph, t0, t_end, dt0, cfl, q, stages=5, max_steps=100000, nthreads=0):
+    """One output frame of Apollo's stepping (TSSSP rks2 under ApSolver's
+    dt control).  Returns (q, dts)."""
+    q = np.array(q, dtype=np.float64, copy=True)
+    dts = np.zeros(max(max_steps, 1))
+    n = ctypes.c_int(0)
+    rc = lib().orc_ssp_advance(ctypes.byref(mesh.ops._o), ctypes.byref(ph), mesh.K, _d(mesh.x),
+                               _d(mesh.y), _i(mesh.etetf), t0, t_end, dt0, cfl, mesh.dtscale,
+                               stages, _d(q), max_steps, ctypes.byref(n), _d(dts), nthreads)
+    if rc:
+        raise ValueError("orc_ssp_advance: %d" % rc)
+    return q, dts[:n.value].copy()
 
 
 def adjoint_rhs(mesh, ph, t, q, lam):

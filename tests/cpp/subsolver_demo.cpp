@@ -1,7 +1,8 @@
 // A C++ caller of the Apollo-shaped interface: sets up the Maxwell TM
 // cavity on a jittered walled mesh, evaluates one RHS through step(), runs
-// LSERK4 steps through advance(), and prints the results for the Python test
-// to compare with the oracle.  Exit status 0 on success.
+// LSERK4 steps through advance() and two output frames of Apollo's own
+// stepping through solve(), and writes the results for the Python test to
+// compare with the oracle.  Exit status 0 on success.
 //   subsolver_demo <n> <N> <nsteps> <out.bin>
 #include <cmath>
 #include <cstdio>
@@ -49,6 +50,11 @@ int main(int argc, char **argv) {
     WxStepperStatus<double> st = dg.step(0.0, dt0, q.data(), rhs.data());
     const double dt = st.getSuggestedDt();
     dg.advance(0.0, dt, nsteps, q.data());
+    // two output frames of Apollo's own stepping from q0, each restarted at dt0
+    std::vector<double> qf = q0;
+    int frames = 0;
+    dg.solve(2 * nsteps * dt, 2, dt0, qf.data(), [&](int, double, const double *) { ++frames; });
+    if (frames != 2) return 5;
     std::FILE *f = std::fopen(argv[4], "wb");
     if (!f) return 4;
     const double hdr[3] = {dt0, dt, (double)len};
@@ -56,6 +62,7 @@ int main(int argc, char **argv) {
     std::fwrite(q0.data(), sizeof(double), len, f);
     std::fwrite(rhs.data(), sizeof(double), len, f);
     std::fwrite(q.data(), sizeof(double), len, f);
+    std::fwrite(qf.data(), sizeof(double), len, f);
     std::fclose(f);
     std::printf("subsolver_demo: K=%u Np=%u m=%u dt0=%.6e dt=%.6e\n", dg.numElements(), dg.NpElem(),
                 dg.totalEqns(), dt0, dt);

@@ -225,8 +225,8 @@ def test_rhs_error_codes():
 
 def test_cpp_subsolver_interface(tmp_path):
     """The Apollo-shaped C++ interface (apollo_amd/host) driven from C++:
-    setup/init/step/advance on the Maxwell TM cavity, checked against the
-    oracle and the Python binding."""
+    setup/init/step/advance/solve on the Maxwell TM cavity, checked against
+    the oracle and the Python binding."""
     import os
     import subprocess
     from cases import Case, walled_unit_square
@@ -240,7 +240,7 @@ def test_cpp_subsolver_interface(tmp_path):
     assert r.returncode == 0
     raw = np.fromfile(str(out), dtype=np.float64)
     dt0, dt, length = raw[0], raw[1], int(raw[2])
-    q0, rhs, qT = raw[3:3 + length], raw[3 + length:3 + 2 * length], raw[3 + 2 * length:]
+    q0, rhs, qT, qF = (raw[3 + i * length:3 + (i + 1) * length] for i in range(4))
     case = Case(walled_unit_square(n), E.Params(N, E.EQN_MAXWELL_TM3, bcs=[(E.BC_MAXWELL_TM, [])]))
     q0 = q0.reshape(case.shape)
     fp64_parity(case, rhs.reshape(case.shape), q0)
@@ -251,3 +251,11 @@ def test_cpp_subsolver_interface(tmp_path):
     q = q0.copy()
     eng.lserk4(0.0, dt, steps, q)
     assert rel_l2(qT.reshape(case.shape), q) <= 1e-14
+    # solve(): two frames of Apollo's stepping, each restarted at dt0
+    from cases import to_oracle, from_oracle
+    qo = to_oracle(case.params.eqn, q0)
+    tsize = 2 * steps * dt / 2
+    for i in range(2):
+        qo, _ = O.ssp_advance(case.omesh, case.ophys, i * tsize, (i + 1) * tsize, dt0, cfl, qo)
+    qo = from_oracle(case.params.eqn, qo)
+    assert rel_l2(qF.reshape(case.shape) - q0, qo - q0) <= 1e-11

@@ -364,6 +364,105 @@ def test_lserk4_order_and_stability_polynomial():
     assert abs(amp(-4.6)) < 1.0 and abs(amp(-4.7)) > 1.0
 
 
+# --- Apollo's per-frame stepping: TSSSP rks2 under ApSolver's dt control -----------
+
+def _dense_rhs_matrix(case):
+    """The (linear) Maxwell RHS of a tiny case as a dense matrix, column by column."""
+    q0 = to_oracle(case.params.eqn, case.state())
+    n = q0.size
+    A = np.zeros((n, n))
+    e = np.zeros(n)
+    for j in range(n):
+        e[j] = 1.0
+        A[:, j] = O.rhs(case.omesh, case.ophys, 0.0, e.reshape(q0.shape))[0].ravel()
+        e[j] = 0.0
+    return A, q0.shape
+
+
+@pytest.mark.parametrize("stages", [2, 5])
+def test_ssp_step_is_the_stability_polynomial(stages):
+    """One rks2 step of a linear RHS u' = A u is u -> R(dt A) u with the
+    method's stability polynomial R(z) = 1/s + (s-1)/s (1 + z/(s-1))^s.
+    Checked on random vectors against powers of the dense DG operator, and on
+    eigenvectors against the scalar R(dt lam) (to the eigenpair's residual)."""
+    from apollo_amd import engine as E
+    case = maxwell_case(n=2, N=2, m=3, bc=E.BC_PEC, flux=E.FLUX_UPWIND)
+    A, shape = _dense_rhs_matrix(case)
+    lam, V = np.linalg.eig(A)
+    dt = 0.8 / np.abs(lam).max()
+    s = stages
+    rng = np.random.default_rng(11)
+    for _ in range(3):
+        x = rng.standard_normal(A.shape[0])
+        y = x.copy()
+        for _ in range(s):
+            y = y + dt / (s - 1.0) * (A @ y)
+        want = x / s + (s - 1.0) / s * y
+        got, dts = O.ssp_advance(case.omesh, case.ophys, 0.0, dt, dt, 1e9, x.reshape(shape),
+                                 stages=s)
+        assert len(dts) == 1 and dts[0] == dt
+        assert np.linalg.norm(got.ravel() - want) <= 1e-12 * np.linalg.norm(x)
+    for k in np.argsort(-np.abs(lam))[:6:2]:
+        v = V[:, k]
+        resid = np.linalg.norm(A @ v - lam[k] * v) / abs(lam[k])
+        R = 1.0 / s + (s - 1.0) / s * (1.0 + dt * lam[k] / (s - 1.0)) ** s
+        use_real = np.linalg.norm(np.real(v)) > 0.5
+        x = np.real(v) if use_real else np.imag(v)
+        part = np.real if use_real else np.imag
+        got, _ = O.ssp_advance(case.omesh, case.ophys, 0.0, dt, dt, 1e9, x.reshape(shape), stages=s)
+        err = np.linalg.norm(got.ravel() - part(R * v)) / np.linalg.norm(x)
+        assert err <= 1e-10 + 50 * s * resid, (err, resid)
+
+
+def test_ssp_second_order_on_cavity_mode():
+    """Fixed dt (cfl large): halving dt divides the error against the analytic
+    TM cavity mode by ~4."""
+    from apollo_amd import engine as E
+    case = maxwell_case(n=6, N=6, m=3, bc=E.BC_PEC, flux=E.FLUX_UPWIND)
+    x, y = case.omesh.x, case.omesh.y
+    q0 = to_oracle(case.params.eqn, E.tm_cavity_mode(x, y, 0.0, p=1, q=1, m=3))
+    T = 0.2
+    errs = []
+    for dt in (4e-3, 2e-3):
+        q, dts = O.ssp_advance(case.omesh, case.ophys, 0.0, T, dt, 1e9, q0)
+        assert len(dts) == round(T / dt)
+        ex = to_oracle(case.params.eqn, E.tm_cavity_mode(x, y, T, p=1, q=1, m=3))
+        errs.append(np.linalg.norm((q - ex).ravel()) / np.linalg.norm(ex.ravel()))
+    assert 3.5 < errs[0] / errs[1] < 4.5, errs
+
+
+def test_ssp_dt_control_constant_speed():
+    """Maxwell (wave speed c0 = 1 everywhere): the first step takes dt0, every
+    later one min(dt, (2/3) cfl dtscale rmin / c0), the last is clamped to the
+    frame end, and the steps sum to the frame."""
+    from apollo_amd import engine as E
+    case = maxwell_case(n=4, N=3, m=3, bc=E.BC_PEC)
+    q0 = to_oracle(case.params.eqn, case.state())
+    cfl = 0.5 / 7
+    sugg = O.suggested_dt(case.omesh, cfl, 1.0)
+    t0, t1 = 0.1, 0.1 + 7.3 * sugg
+    q, dts = O.ssp_advance(case.omesh, case.ophys, t0, t1, 2.5 * sugg, cfl, q0)
+    assert dts[0] == 2.5 * sugg
+    assert np.all(dts[1:-1] == sugg)
+    assert 0 < dts[-1] <= sugg
+    assert abs(t0 + dts.sum() - t1) <= 1e-15 * t1
+    assert len(dts) == 1 + int(np.ceil((t1 - t0 - 2.5 * sugg) / sugg - 1e-12))
+
+
+def test_ssp_dt_control_nonincreasing_euler():
+    """Euler vortex: dt never grows within a frame, and shrinks below dt0
+    when dt0 exceeds the suggested step."""
+    case = euler_case(n=6, N=3, m=4)
+    q0 = to_oracle(case.params.eqn, case.state())
+    r, ms = O.rhs(case.omesh, case.ophys, 0.0, q0)
+    s0 = O.suggested_dt(case.omesh, 0.5, ms)
+    q, dts = O.ssp_advance(case.omesh, case.ophys, 0.0, 6 * s0, 3 * s0, 0.5, q0)
+    assert dts[0] == 3 * s0
+    assert np.all(np.diff(dts) <= 0.0)
+    assert dts[1] < dts[0]
+    assert abs(dts.sum() - 6 * s0) <= 1e-14 * s0
+
+
 # --- adjoint ---------------------------------------------------------------------
 
 @pytest.mark.parametrize("kind", ["euler", "maxwell_tm", "maxwell_pec_upwind", "vortex_bc"])
