@@ -309,7 +309,7 @@ __global__ void __launch_bounds__((64 * AdjFaceGroup<N, EQN>::W)) adj_face_kerne
   if (a.mode == 1 && wb.active)
     axpy_stage_transposed<ES>(LT, rb + e0 * ES, a.qbar + e0 * ES, a.rk_b, cnt, lane);
   else
-    stage_transposed<ES>(LT, rb + e0 * ES, cnt, lane);
+    stage_transposed<ES, 16>(LT, rb + e0 * ES, cnt, lane);
   for (int i = 16 * ES + lane; i < NKS1 * 4 * M * 16; i += 64) LT[i] = 0.0;
   wave_sync();
   double b[NKS1][M];
@@ -452,8 +452,8 @@ __global__ void __launch_bounds__((64 * AdjGroup<N, EQN>::W), (AdjWaves<N, EQN>:
   const double sc = el < cnt ? gk[G_INVJ] : 0.0;
   for (int i = lane; i < L::I1D; i += 64) I1[i] = a.I1D[i];
   for (int i = lane; i < 3 * Nfp; i += 64) FM[i] = a.fmask[i];
-  stage_transposed<ES>(XT, a.q + e0 * ES, cnt, lane);
-  stage_transposed<ES>(LT, a.lam + e0 * ES, cnt, lane);
+  stage_transposed<ES, 16>(XT, a.q + e0 * ES, cnt, lane);
+  stage_transposed<ES, 16>(LT, a.lam + e0 * ES, cnt, lane);
   for (int i = 16 * ES + lane; i < L::XT; i += 64) XT[i] = LT[i] = 0.0;
   wave_sync();
   for (int it = lane; it < 16 * 3 * Nfp; it += 64) {

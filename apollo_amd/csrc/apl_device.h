@@ -52,6 +52,11 @@ struct Order {
   static constexpr int NKF = (3 * Ng + 3) / 4;   // k-steps of the face GEMM
 };
 
+// Row stride (doubles) of the transposed batch state in LDS: 16 elements plus
+// one pad column, so that the staging writes (consecutive lanes two rows
+// apart) spread over the LDS banks instead of all hitting one.
+constexpr int XS = 17;
+
 // rows of the transposed batch state staged in LDS: the K extent of the
 // first contraction, 4-row steps for FP32/FP64 MFMA and 32-row steps for
 // BF16 (rows >= Np are zero)
@@ -407,10 +412,10 @@ __device__ __forceinline__ WaveBatch wave_batch(int K) {
 
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 
-// XT[w*16 + e] = q[(e0 + e)*ES + w]: all loads of a full batch are issued
+// XT[w*RS + e] = q[(e0 + e)*ES + w]: all loads of a full batch are issued
 // before any LDS write (16-byte vector loads; the batch base is 128-byte
 // aligned because 16*ES*8 is a multiple of 128).
-template <int ES>
+template <int ES, int RS = XS>
 __device__ __forceinline__ void stage_transposed(double *XT, const double *qb, int cnt, int lane) {
   constexpr int NV2 = 16 * ES / 2;
   constexpr int NIT = (NV2 + 63) / 64;
@@ -428,14 +433,14 @@ __device__ __forceinline__ void stage_transposed(double *XT, const double *qb, i
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const int i = 2 * idx + h, e = i / ES, w = i - e * ES;
-          XT[w * 16 + e] = buf[j][h];
+          XT[w * RS + e] = buf[j][h];
         }
       }
     }
   } else {
     for (int i = lane; i < 16 * ES; i += 64) {
       const int e = i / ES, w = i - e * ES;
-      XT[w * 16 + e] = (e < cnt) ? qb[i] : qb[w];
+      XT[w * RS + e] = (e < cnt) ? qb[i] : qb[w];
     }
   }
 }
@@ -541,8 +546,8 @@ __device__ __forceinline__ unsigned write_batch(const RhsArgs &a, const double *
 // the fused RHS kernel
 // ---------------------------------------------------------------------------
 
-// LDS of one single-wave workgroup (doubles).  The batch state is staged
-// transposed, XT[(node*M + v)*16 + e], so that B-operand and face-node reads
+// LDS of one wave (doubles).  The batch state is staged transposed,
+// XT[(node*M + v)*XS + e], so that B-operand and face-node reads
 // of the 16 lanes of a lane group hit consecutive addresses (the own face
 // nodes are read from it through Fmask); the neighbours' face nodes are
 // staged as PT[((f*Nfp + i)*M + v)*16 + e] in the own face's node order
@@ -552,7 +557,7 @@ template <int N, int EQN, typename T>
 struct Lds {
   static constexpr int M = Vars<EQN>::M;
   static constexpr int Np = Order<N>::Np, Nfp = Order<N>::Nfp, Ng = Order<N>::Ng;
-  static constexpr int XT = xt_rows<N, T>() * M * 16;   // rows >= Np are zero
+  static constexpr int XT = xt_rows<N, T>() * M * XS;   // rows >= Np are zero
   static constexpr int PT = 3 * Nfp * M * 16;
   static constexpr int I1D = 3 * Ng * Nfp;
   static constexpr int FM = (3 * Nfp + 1) / 2;   // ints packed into doubles
@@ -597,9 +602,9 @@ __device__ __forceinline__ void face_flux_cub(const RhsArgs &a, int ks, S *fsT, 
 #pragma unroll
     for (int i = 0; i < Nfp; ++i) {
       const double w = Io[i];
-      const double *xo = F.XT + F.FM[f * Nfp + i] * M * 16 + F.el;
+      const double *xo = F.XT + F.FM[f * Nfp + i] * M * XS + F.el;
 #pragma unroll
-      for (int v = 0; v < M; ++v) qM[v] += w * xo[v * 16];
+      for (int v = 0; v < M; ++v) qM[v] += w * xo[v * XS];
     }
     if (code >= 0) {
       // own node i <-> neighbour node Fmask[nf][Nfp-1-i]; the neighbour's
@@ -643,7 +648,7 @@ __device__ __forceinline__ void split_state(const double *XT, int lg, int el, bf
     for (int it = 0; it < 8; ++it)
 #pragma unroll
       for (int v = 0; v < M; ++v)
-        split_bf16(float(XT[((32 * kk + 4 * it + lg) * M + v) * 16 + el]), qh[kk][v], ql[kk][v], it);
+        split_bf16(float(XT[((32 * kk + 4 * it + lg) * M + v) * XS + el]), qh[kk][v], ql[kk][v], it);
 }
 
 // Waves per SIMD the register allocator is asked to allow: two where the
@@ -710,7 +715,7 @@ __global__ void __launch_bounds__((64 * RhsGroup<N, EQN, T>::W), (MinWaves<N, EQ
 
   // 2. stage the batch state, transposed (coalesced global reads)
   stage_transposed<ES>(XT, a.q + e0 * ES, cnt, lane);
-  for (int i = 16 * ES + lane; i < L::XT; i += 64) XT[i] = 0.0;
+  for (int i = ES * XS + lane; i < L::XT; i += 64) XT[i] = 0.0;
   wave_sync();
 
   // 3. gather the neighbours' face nodes; item (k = f*Nfp + i, e), e fastest
@@ -801,7 +806,7 @@ __global__ void __launch_bounds__((64 * RhsGroup<N, EQN, T>::W), (MinWaves<N, EQ
       for (int ks = 0; ks < NKS1; ++ks) {
         a1[ks] = A1[(cc * NKS1 + ks) * 64 + lane];
 #pragma unroll
-        for (int v = 0; v < M; ++v) b1[ks][v] = S(XT[((4 * ks + lg) * M + v) * 16 + el]);
+        for (int v = 0; v < M; ++v) b1[ks][v] = S(XT[((4 * ks + lg) * M + v) * XS + el]);
       }
 #pragma unroll
       for (int rt = 0; rt < NRT; ++rt)
@@ -965,7 +970,7 @@ __global__ void __launch_bounds__((64 * RhsGroup<N, EQN, T>::W), (MinWaves<N, EQ
 template <int N, int EQN, typename T>
 struct LinLds {
   static constexpr int M = Vars<EQN>::M;
-  static constexpr int XT = xt_rows<N, T>() * M * 16;
+  static constexpr int XT = xt_rows<N, T>() * M * XS;
   static constexpr int doubles = XT;
 };
 
@@ -1070,7 +1075,7 @@ __global__ void __launch_bounds__((64 * LinGroup<N, EQN, T>::W), (LinWaves<N, EQ
   }
 
   stage_transposed<ES>(XT, a.q + e0 * ES, cnt, lane);
-  for (int i = 16 * ES + lane; i < L::XT; i += 64) XT[i] = 0.0;
+  for (int i = ES * XS + lane; i < L::XT; i += 64) XT[i] = 0.0;
   wave_sync();
 
   unsigned ef = 0;
@@ -1105,7 +1110,7 @@ __global__ void __launch_bounds__((64 * LinGroup<N, EQN, T>::W), (LinWaves<N, EQ
       double *qp = qP[PRE ? ks : 0];
       if constexpr (!PRE) gather_face_node<Np, Nfp, M>(qg, kf, code0, code1, code2, fnb[ks], qp);
 #pragma unroll
-      for (int v = 0; v < M; ++v) qM[v] = XT[(fown[ks] * M + v) * 16 + el];
+      for (int v = 0; v < M; ++v) qM[v] = XT[(fown[ks] * M + v) * XS + el];
       if (code < 0) boundary_state<EQN>(a.bc, -code - 1, 0.0, 0.0, 0.0, nx, ny, qM, qp);
       double lam;
       ef |= num_flux<EQN, FLUX>(a.ph, nx, ny, qM, qp, fs, &lam);
@@ -1151,7 +1156,7 @@ __global__ void __launch_bounds__((64 * LinGroup<N, EQN, T>::W), (LinWaves<N, EQ
 #pragma unroll
     for (int ks = 0; ks < NKS1; ++ks)
 #pragma unroll
-      for (int v = 0; v < M; ++v) b1[ks][v] = S(XT[((4 * ks + lg) * M + v) * 16 + el]);
+      for (int v = 0; v < M; ++v) b1[ks][v] = S(XT[((4 * ks + lg) * M + v) * XS + el]);
   }
   // BF16x3: the state split once into hi/lo B operands
   bf8 bh[BF ? NK8 : 1][M], bl[BF ? NK8 : 1][M];
@@ -1162,7 +1167,7 @@ __global__ void __launch_bounds__((64 * LinGroup<N, EQN, T>::W), (LinWaves<N, EQ
       for (int it = 0; it < 8; ++it)
 #pragma unroll
         for (int v = 0; v < M; ++v)
-          split_bf16(float(XT[((32 * kk + 4 * it + lg) * M + v) * 16 + el]), bh[kk][v], bl[kk][v], it);
+          split_bf16(float(XT[((32 * kk + 4 * it + lg) * M + v) * XS + el]), bh[kk][v], bl[kk][v], it);
   }
 
 #pragma unroll
@@ -1187,7 +1192,7 @@ __global__ void __launch_bounds__((64 * LinGroup<N, EQN, T>::W), (LinWaves<N, EQ
         for (int ks = 0; ks < NKS1; ++ks)
 #pragma unroll
           for (int v = 0; v < M; ++v) {
-            const S b = KEEP_B ? b1[KEEP_B ? ks : 0][v] : S(XT[((4 * ks + lg) * M + v) * 16 + el]);
+            const S b = KEEP_B ? b1[KEEP_B ? ks : 0][v] : S(XT[((4 * ks + lg) * M + v) * XS + el]);
             p[v] = mma(ad[ks], b, p[v]);
           }
       }
