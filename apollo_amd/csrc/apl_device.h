@@ -195,8 +195,9 @@ __device__ __forceinline__ unsigned phys_flux(const PhysParams &P, const S *q, S
     S rho = q[0];
     if (P.min_dens == 0.0 && !(rho > S(0))) ef |= EF_DENSITY;
     if (rho < S(P.min_dens)) rho = S(P.min_dens);
-    const S u = q[1] / rho, v = q[2] / rho;
-    const S w = W ? q[3] / rho : S(0);
+    const S ir = S(1) / rho;   // one division for the three velocities
+    const S u = q[1] * ir, v = q[2] * ir;
+    const S w = W ? q[3] * ir : S(0);
     const S E = q[IE];
     S p = S(P.gamma - 1.0) * (E - S(0.5) * rho * (u * u + v * v + w * w));
     if (!(p > S(0))) {
@@ -230,11 +231,11 @@ template <int EQN>
 __device__ __forceinline__ double euler_speed(const PhysParams &P, const double *q) {
   constexpr bool W = (EQN == APL_EQN_EULER5);
   constexpr int IE = W ? 4 : 3;
-  const double r = q[0];
+  const double ir = 1.0 / q[0];
   const double mw = W ? q[3] : 0.0;
-  const double p = (P.gamma - 1.) * (q[IE] - 0.5 * (q[1] * q[1] + q[2] * q[2] + mw * mw) / r);
-  const double u = q[1] / r, v = q[2] / r, w = mw / r;
-  return sqrt(u * u + v * v + w * w) + sqrt(P.gamma * p / r);
+  const double p = (P.gamma - 1.) * (q[IE] - 0.5 * (q[1] * q[1] + q[2] * q[2] + mw * mw) * ir);
+  const double u = q[1] * ir, v = q[2] * ir, w = mw * ir;
+  return sqrt(u * u + v * v + w * w) + sqrt(P.gamma * p * ir);
 }
 
 // Numerical flux F*(n, qM, qP): LF (wxeulereqn.cc:1088-1117,
@@ -779,7 +780,7 @@ __global__ void __launch_bounds__((64 * RhsGroup<N, EQN, T>::W), (MinWaves<N, EQ
     }
   }
 
-#pragma unroll 1
+#pragma unroll 2
   for (int cc = 0; cc < NCH; ++cc) {
     V4 qc[M];
 #pragma unroll
@@ -905,7 +906,7 @@ __global__ void __launch_bounds__((64 * RhsGroup<N, EQN, T>::W), (MinWaves<N, EQ
       }
     }
   } else {
-#pragma unroll 1
+#pragma unroll 2
     for (int ks = 0; ks < NKF; ++ks) {
       S a3[NRT];
 #pragma unroll
