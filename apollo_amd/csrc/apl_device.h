@@ -238,14 +238,113 @@ __device__ __forceinline__ double euler_speed(const PhysParams &P, const double 
   return sqrt(u * u + v * v + w * w) + sqrt(P.gamma * p * ir);
 }
 
+// HLL (wxeulereqn.cc:1119-1202) and Roe (:1204-1268) fluxes of the Euler
+// equations in the face's (normal, tangential) frame, in Apollo's order of
+// steps.  Roe is reproduced as Apollo computes it: the back-rotation assigns
+// the x-momentum slot twice, so it holds the y-momentum flux, and the
+// y-momentum slot keeps the 0 its buffer starts with.  Both zero the z
+// momentum flux.  *lam = max(|u| + c) of the two traces.
+template <int EQN, int FLUX>
+__device__ __forceinline__ unsigned euler_riemann_flux(const PhysParams &P, double nx, double ny,
+                                                       const double *qMi, const double *qPi,
+                                                       double *fs, double *lam) {
+  constexpr int M = Vars<EQN>::M;
+  constexpr bool W = (EQN == APL_EQN_EULER5);
+  constexpr int IE = W ? 4 : 3;
+  const double g = P.gamma;
+  double qM[M], qP[M];
+#pragma unroll
+  for (int c = 0; c < M; ++c) {
+    qM[c] = qMi[c];
+    qP[c] = qPi[c];
+  }
+  qM[1] = nx * qMi[1] + ny * qMi[2];
+  qM[2] = -ny * qMi[1] + nx * qMi[2];
+  qP[1] = nx * qPi[1] + ny * qPi[2];
+  qP[2] = -ny * qPi[1] + nx * qPi[2];
+  double FM[M], GM[M], FP[M], GP[M];
+  unsigned ef = phys_flux<EQN, double>(P, qM, FM, GM);
+  ef |= phys_flux<EQN, double>(P, qP, FP, GP);
+  // rotated primitives: rho, u (normal), v, w, p
+  const double iM = 1.0 / qM[0], iP = 1.0 / qP[0];
+  const double uM = qM[1] * iM, vM = qM[2] * iM, wM = W ? qM[3] * iM : 0.0;
+  const double uP = qP[1] * iP, vP = qP[2] * iP, wP = W ? qP[3] * iP : 0.0;
+  const double mwM = W ? qM[3] : 0.0, mwP = W ? qP[3] : 0.0;
+  const double pM = (g - 1.) * (qM[IE] - 0.5 * (qM[1] * qM[1] + qM[2] * qM[2] + mwM * mwM) * iM);
+  const double pP = (g - 1.) * (qP[IE] - 0.5 * (qP[1] * qP[1] + qP[2] * qP[2] + mwP * mwP) * iP);
+  const double cM = sqrt(g * pM * iM), cP = sqrt(g * pP * iP);
+  // Roe averages
+  const double HM = (qM[IE] + pM) * iM, HP = (qP[IE] + pP) * iP;
+  const double rM = sqrt(qM[0]), rP = sqrt(qP[0]), irs = 1.0 / (rM + rP);
+  const double u = (rM * uM + rP * uP) * irs, v = (rM * vM + rP * vP) * irs;
+  const double w = (rM * wM + rP * wP) * irs, H = (rM * HM + rP * HP) * irs;
+  const double c2 = (g - 1.) * (H - 0.5 * (u * u + v * v + w * w)), c = sqrt(c2);
+  double fx[M];
+  if constexpr (FLUX == APL_FLUX_HLL) {
+    const double SL = fmin(uM - cM, u - c), SR = fmax(uP + cP, u + c);
+    const double t1 = (fmin(SR, 0.0) - fmin(0.0, SL)) / (SR - SL);
+    const double t2 = 1. - t1;
+    const double t3 = (SR * fabs(SL) - SL * fabs(SR)) / (2. * (SR - SL));
+#pragma unroll
+    for (int k = 0; k < M; ++k) fx[k] = t1 * FP[k] + t2 * FM[k] - t3 * (qP[k] - qM[k]);
+    fs[1] = nx * fx[1] - ny * fx[2];
+    fs[2] = ny * fx[1] + nx * fx[2];
+  } else {
+    const double rho = rM * rP;
+    const double dp = pP - pM;
+    double dw1 = -0.5 * rho * (uP - uM) / c + 0.5 * dp / c2;
+    double dw2 = (qP[0] - qM[0]) - dp / c2;
+    double dw3 = rho * (vP - vM);
+    double dw4 = 0.5 * rho * (uP - uM) / c + 0.5 * dp / c2;
+    dw1 = fabs(u - c) * dw1;
+    dw2 = fabs(u) * dw2;
+    dw3 = fabs(u) * dw3;
+    dw4 = fabs(u + c) * dw4;
+#pragma unroll
+    for (int k = 0; k < M; ++k) fx[k] = 0.5 * (FM[k] + FP[k]);
+    fx[0] -= 0.5 * (dw1 + dw2 + dw4);
+    fx[1] -= 0.5 * (dw1 * (u - c) + dw2 * u + dw4 * (u + c));
+    fx[2] -= 0.5 * (dw1 * v + dw2 * v + dw3 + dw4 * v);
+    fx[IE] -= 0.5 * (dw1 * (H - u * c) + dw2 * (u * u + v * v) / 2. + dw3 * v + dw4 * (H + u * c));
+    fs[1] = ny * fx[1] + nx * fx[2];
+    fs[2] = 0.0;
+  }
+  fs[0] = fx[0];
+  if constexpr (W) fs[3] = 0.0;
+  fs[IE] = fx[IE];
+  const double lM = sqrt(uM * uM + vM * vM + wM * wM) + cM;
+  const double lP = sqrt(uP * uP + vP * vP + wP * wP) + cP;
+  *lam = lM > lP ? lM : lP;
+  return ef;
+}
+
 // Numerical flux F*(n, qM, qP): LF (wxeulereqn.cc:1088-1117,
-// wxphmaxwelleqn.cc:500-522) or, for TM3, the exact upwind flux (LF minus
-// the dissipation of the normal-B jump).  Returns flags; *lam = LF speed.
+// wxphmaxwelleqn.cc:500-522), Euler HLL / Roe / Wave, or, for TM3, the exact
+// upwind flux (LF minus the dissipation of the normal-B jump).  Returns
+// flags; *lam = the fastest wave speed.
 template <int EQN, int FLUX>
 __device__ __forceinline__ unsigned num_flux(const PhysParams &P, double nx, double ny,
                                              const double *qM, const double *qP, double *fs,
                                              double *lam) {
   constexpr int M = Vars<EQN>::M;
+  if constexpr ((EQN == APL_EQN_EULER5 || EQN == APL_EQN_EULER4) &&
+                (FLUX == APL_FLUX_HLL || FLUX == APL_FLUX_ROE))
+    return euler_riemann_flux<EQN, FLUX>(P, nx, ny, qM, qP, fs, lam);
+  if constexpr ((EQN == APL_EQN_EULER5 || EQN == APL_EQN_EULER4) && FLUX == APL_FLUX_WAVE) {
+    // Apollo's "Wave": momentum of both traces rotated to the face frame,
+    // then LF with the Cartesian normal.
+    double rM[M], rP[M];
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+      rM[c] = qM[c];
+      rP[c] = qP[c];
+    }
+    rM[1] = nx * qM[1] + ny * qM[2];
+    rM[2] = -ny * qM[1] + nx * qM[2];
+    rP[1] = nx * qP[1] + ny * qP[2];
+    rP[2] = -ny * qP[1] + nx * qP[2];
+    return num_flux<EQN, APL_FLUX_LF>(P, nx, ny, rM, rP, fs, lam);
+  }
   double FM[M], GM[M], FP[M], GP[M];
   unsigned ef = phys_flux<EQN, double>(P, qM, FM, GM);
   ef |= phys_flux<EQN, double>(P, qP, FP, GP);

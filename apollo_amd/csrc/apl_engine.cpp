@@ -453,9 +453,13 @@ int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out)
   if (params->order < 1 || params->order > 8) return fail(nullptr, APL_ERR_ORDER, "order outside 1..8");
   const int M = nvars(params->eqn);
   if (!M) return fail(nullptr, APL_ERR_ARG, "unknown equation kind");
+  const bool euler = params->eqn == APL_EQN_EULER5 || params->eqn == APL_EQN_EULER4;
   if (params->flux != APL_FLUX_LF &&
-      !(params->flux == APL_FLUX_UPWIND && params->eqn == APL_EQN_MAXWELL_TM3))
-    return fail(nullptr, APL_ERR_ARG, "upwind flux is available for MAXWELL_TM3 only");
+      !(params->flux == APL_FLUX_UPWIND && params->eqn == APL_EQN_MAXWELL_TM3) &&
+      !((params->flux == APL_FLUX_HLL || params->flux == APL_FLUX_ROE ||
+         params->flux == APL_FLUX_WAVE) && euler))
+    return fail(nullptr, APL_ERR_ARG,
+                "flux: upwind is for MAXWELL_TM3 only, HLL, Roe and Wave for the Euler equations only");
   if (params->precision != APL_PREC_FP64 && params->precision != APL_PREC_FP32_MFMA &&
       params->precision != APL_PREC_BF16X3)
     return fail(nullptr, APL_ERR_ARG, "unknown precision");
@@ -1034,6 +1038,8 @@ static AdjArgs adj_args(apl_handle *h) {
 static int dispatch_adjoint(apl_handle *h, const AdjArgs &a) {
   const int nb = (h->K + 15) / 16;
   const apl_params &p = h->params;
+  if (p.flux == APL_FLUX_HLL || p.flux == APL_FLUX_ROE || p.flux == APL_FLUX_WAVE)
+    return fail(h, APL_ERR_UNSUPPORTED, "the adjoint covers the Lax-Friedrichs and upwind fluxes");
   int rc;
   switch (p.order) {
     case 1: rc = launch_adjoint<1>(p.eqn, p.flux, h->stream, a, nb); break;

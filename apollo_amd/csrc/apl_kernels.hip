@@ -55,8 +55,14 @@ int launch_rhs(int eqn, int flux, int prec, int linear, hipStream_t s, const Rhs
                int nblocks) {
   switch (eqn) {
     case APL_EQN_EULER5:
+      if (flux == APL_FLUX_HLL) return launch_prec<N, APL_EQN_EULER5, APL_FLUX_HLL>(prec, linear, s, a, nblocks);
+      if (flux == APL_FLUX_ROE) return launch_prec<N, APL_EQN_EULER5, APL_FLUX_ROE>(prec, linear, s, a, nblocks);
+      if (flux == APL_FLUX_WAVE) return launch_prec<N, APL_EQN_EULER5, APL_FLUX_WAVE>(prec, linear, s, a, nblocks);
       return launch_prec<N, APL_EQN_EULER5, APL_FLUX_LF>(prec, linear, s, a, nblocks);
     case APL_EQN_EULER4:
+      if (flux == APL_FLUX_HLL) return launch_prec<N, APL_EQN_EULER4, APL_FLUX_HLL>(prec, linear, s, a, nblocks);
+      if (flux == APL_FLUX_ROE) return launch_prec<N, APL_EQN_EULER4, APL_FLUX_ROE>(prec, linear, s, a, nblocks);
+      if (flux == APL_FLUX_WAVE) return launch_prec<N, APL_EQN_EULER4, APL_FLUX_WAVE>(prec, linear, s, a, nblocks);
       return launch_prec<N, APL_EQN_EULER4, APL_FLUX_LF>(prec, linear, s, a, nblocks);
     case APL_EQN_PHMAXWELL8:
       return launch_prec<N, APL_EQN_PHMAXWELL8, APL_FLUX_LF>(prec, linear, s, a, nblocks);

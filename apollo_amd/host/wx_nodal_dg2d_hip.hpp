@@ -58,7 +58,10 @@ struct NodalDG2dConfig {
   int polynomialOrder = 3;
   double cfl = 0.5;
   std::string equation = "eulerEqn";     // "eulerEqn" | "phMaxwellEqn" | "euler2d" | "maxwellTM"
-  std::string numericalFlux = "lax";     // "lax" | "upwind" (maxwellTM only)
+  // Euler "Numerical_Flux" (wxeulereqn.cc:44-84): "LF" | "HLL" | "Roe" |
+  // "Wave"; any other name, or none, selects LF as Apollo does.  "upwind"
+  // selects the exact TM upwind flux (maxwellTM only).
+  std::string numericalFlux = "LF";
   double gas_gamma = 1.4, minDensity = 0.0, minPressure = 0.0;
   double c0 = 1.0, chi = 0.0, gamma = 0.0;
   int precision = APL_PREC_FP64;
@@ -98,7 +101,7 @@ class WxNodalDG2dMethodHip {
     apl_params p{};
     p.order = cfg.polynomialOrder;
     p.eqn = equation_kind(cfg.equation);
-    p.flux = cfg.numericalFlux == "upwind" ? APL_FLUX_UPWIND : APL_FLUX_LF;
+    p.flux = flux_kind(cfg.numericalFlux);
     p.precision = cfg.precision;
     p.algorithm = cfg.algorithm;
     p.gas_gamma = cfg.gas_gamma;
@@ -190,6 +193,13 @@ class WxNodalDG2dMethodHip {
   }
 
  private:
+  static int flux_kind(const std::string &f) {
+    if (f == "HLL") return APL_FLUX_HLL;
+    if (f == "Roe") return APL_FLUX_ROE;
+    if (f == "Wave") return APL_FLUX_WAVE;
+    if (f == "upwind") return APL_FLUX_UPWIND;
+    return APL_FLUX_LF;
+  }
   static int equation_kind(const std::string &e) {
     if (e == "eulerEqn") return APL_EQN_EULER5;
     if (e == "phMaxwellEqn") return APL_EQN_PHMAXWELL8;

@@ -57,8 +57,19 @@ enum apl_eqn {
 enum apl_flux {
   APL_FLUX_LF = 0,         /* Lax-Friedrichs (Rusanov): wxeulereqn.cc:1088-1117,
                               wxphmaxwelleqn.cc:500-522                      */
-  APL_FLUX_UPWIND = 1      /* exact upwind flux, APL_EQN_MAXWELL_TM3 only (no
+  APL_FLUX_UPWIND = 1,     /* exact upwind flux, APL_EQN_MAXWELL_TM3 only (no
                               counterpart in the reference)                  */
+  APL_FLUX_HLL = 2,        /* Euler HLL, wxeulereqn.cc:1119-1202             */
+  APL_FLUX_ROE = 3,        /* Euler Roe as Apollo computes it,
+                              wxeulereqn.cc:1204-1268: the x-momentum slot
+                              holds the y-momentum flux and the y-momentum
+                              flux is 0 (:1259-1260 assign nflux[1] twice)   */
+  APL_FLUX_WAVE = 4        /* Euler "Wave" as Apollo computes it: the wave-
+                              propagation routine rotates both traces'
+                              momentum to the face frame in place
+                              (wxeulereqn.cc:1292-1300) and the switch falls
+                              through to LF (:888-892), so the flux is LF of
+                              the rotated traces                             */
 };
 
 enum apl_precision {

@@ -497,9 +497,122 @@ static void euler_primitive(const orc_phys *ph, double *qc, double *qp) {
 static double dmax2(double a, double b) { return a > b ? a : b; }          /* wxmath.h:15 */
 static double dmax3(double a, double b, double c) { double m1 = a > b ? a : b; return m1 > c ? m1 : c; }
 
+static double dmin2(double a, double b) { return a < b ? a : b; }          /* wxmath.h:28 */
+
+/* Momentum rotated in place to (normal, tangential) components, as
+ * applyHLLFluxes / applyRoeFluxes do to their arguments (wxeulereqn.cc:
+ * 1136-1143, 1214-1221); the RHS passes copies (wxnodaldg2dmethod.cc:450-452). */
+static void rotate_momentum(const double *n, double *q) {
+  double ru = q[1], rv = q[2];
+  q[1] = n[0] * ru + n[1] * rv;
+  q[2] = -n[1] * ru + n[0] * rv;
+}
+
+/* Roe averages of the rotated primitive states (wxeulereqn.cc:1160-1170,
+ * 1232-1243): out = u, v, w, H, c2, rho. */
+static void roe_average(const orc_phys *ph, const double *qM, const double *pM, const double *qP,
+                        const double *pP, double *out) {
+  double HM = (qM[4] + pM[4]) / pM[0], HP = (qP[4] + pP[4]) / pP[0];
+  double rhoMs = sqrt(pM[0]), rhoPs = sqrt(pP[0]);
+  double u = (rhoMs * pM[1] + rhoPs * pP[1]) / (rhoMs + rhoPs);
+  double v = (rhoMs * pM[2] + rhoPs * pP[2]) / (rhoMs + rhoPs);
+  double w = (rhoMs * pM[3] + rhoPs * pP[3]) / (rhoMs + rhoPs);
+  double H = (rhoMs * HM + rhoPs * HP) / (rhoMs + rhoPs);
+  out[0] = u;
+  out[1] = v;
+  out[2] = w;
+  out[3] = H;
+  out[4] = (ph->gamma - 1.) * (H - 0.5 * (u * u + v * v + w * w));
+  out[5] = rhoMs * rhoPs;
+}
+
+/* WxEulerEqn::applyHLLFluxes (wxeulereqn.cc:1119-1202).  `abs(SR)` at :1183
+ * is taken as the floating-point absolute value. */
+static int euler_hll(const orc_phys *ph, const double *n, double *qM, double *qP, double *nflux,
+                     double *lambda) {
+  int rc;
+  double fM[5], fP[5], gM[5], gP[5], pM[5], pP[5], avg[6], fx[5];
+  rotate_momentum(n, qM);
+  rotate_momentum(n, qP);
+  if ((rc = orc_flux(ph, 0, qM, fM)) || (rc = orc_flux(ph, 0, qP, fP)) ||
+      (rc = orc_flux(ph, 1, qM, gM)) || (rc = orc_flux(ph, 1, qP, gP)))
+    return rc;
+  euler_primitive(ph, qM, pM);
+  euler_primitive(ph, qP, pP);
+  double cM = sqrt(ph->gamma * pM[4] / pM[0]), cP = sqrt(ph->gamma * pP[4] / pP[0]);
+  roe_average(ph, qM, pM, qP, pP, avg);
+  double u = avg[0], c = sqrt(avg[4]);
+  double SL = dmin2(pM[1] - cM, u - c), SR = dmax2(pP[1] + cP, u + c);
+  double t1 = (dmin2(SR, 0.0) - dmin2(0.0, SL)) / (SR - SL);
+  double t2 = 1. - t1;
+  double t3 = (SR * fabs(SL) - SL * fabs(SR)) / (2. * (SR - SL));
+  for (int c5 = 0; c5 < 5; c5++) fx[c5] = t1 * fP[c5] + t2 * fM[c5] - t3 * (qP[c5] - qM[c5]);
+  nflux[0] = fx[0];
+  nflux[1] = n[0] * fx[1] - n[1] * fx[2];
+  nflux[2] = n[1] * fx[1] + n[0] * fx[2];
+  nflux[3] = 0.0;
+  nflux[4] = fx[4];
+  double c0M = sqrt(pM[1] * pM[1] + pM[2] * pM[2] + pM[3] * pM[3]) + cM;
+  double c0P = sqrt(pP[1] * pP[1] + pP[2] * pP[2] + pP[3] * pP[3]) + cP;
+  *lambda = dmax2(c0M, c0P);
+  return 0;
+}
+
+/* WxEulerEqn::applyRoeFluxes (wxeulereqn.cc:1204-1268), as written: the
+ * back-rotation assigns nflux[1] twice (:1259-1260), so the x-momentum slot
+ * receives the y-momentum flux n_y fx1 + n_x fx2, and nflux[2] is never
+ * written.  Apollo passes a buffer allocated zero-filled at setup
+ * (wxnodaldg2dmethod.cc:98, wxmath.h:49-55) that only this call writes, so
+ * the y-momentum slot stays 0. */
+static int euler_roe(const orc_phys *ph, const double *n, double *qM, double *qP, double *nflux,
+                     double *lambda) {
+  int rc;
+  double fM[5], fP[5], pM[5], pP[5], avg[6], fx[5];
+  rotate_momentum(n, qM);
+  rotate_momentum(n, qP);
+  euler_primitive(ph, qM, pM);
+  euler_primitive(ph, qP, pP);
+  roe_average(ph, qM, pM, qP, pP, avg);
+  double u = avg[0], v = avg[1], H = avg[3], c2 = avg[4], c = sqrt(c2), rho = avg[5];
+  double dw1 = -0.5 * rho * (pP[1] - pM[1]) / c + 0.5 * (pP[4] - pM[4]) / c2;
+  double dw2 = (pP[0] - pM[0]) - (pP[4] - pM[4]) / c2;
+  double dw3 = rho * (pP[2] - pM[2]);
+  double dw4 = 0.5 * rho * (pP[1] - pM[1]) / c + 0.5 * (pP[4] - pM[4]) / c2;
+  dw1 = fabs(u - c) * dw1;
+  dw2 = fabs(u) * dw2;
+  dw3 = fabs(u) * dw3;
+  dw4 = fabs(u + c) * dw4;
+  if ((rc = orc_flux(ph, 0, qM, fM)) || (rc = orc_flux(ph, 0, qP, fP))) return rc;
+  for (int c5 = 0; c5 < 5; c5++) fx[c5] = 0.5 * (fM[c5] + fP[c5]);
+  fx[0] -= 0.5 * (dw1 * 1. + dw2 * 1. + dw3 * 0. + dw4 * 1.);
+  fx[1] -= 0.5 * (dw1 * (u - c) + dw2 * u + dw3 * 0. + dw4 * (u + c));
+  fx[2] -= 0.5 * (dw1 * v + dw2 * v + dw3 * 1. + dw4 * v);
+  fx[3] = 0.0;
+  fx[4] -= 0.5 * (dw1 * (H - u * c) + dw2 * (u * u + v * v) / 2. + dw3 * v + dw4 * (H + u * c));
+  nflux[0] = fx[0];
+  nflux[1] = n[1] * fx[1] + n[0] * fx[2];
+  nflux[2] = 0.0;
+  nflux[3] = 0.0;
+  nflux[4] = fx[4];
+  double c0M = sqrt(pM[1] * pM[1] + pM[2] * pM[2] + pM[3] * pM[3]) + sqrt(ph->gamma * pM[4] / pM[0]);
+  double c0P = sqrt(pP[1] * pP[1] + pP[2] * pP[2] + pP[3] * pP[3]) + sqrt(ph->gamma * pP[4] / pP[0]);
+  *lambda = dmax2(c0M, c0P);
+  return 0;
+}
+
 int orc_numflux(const orc_phys *ph, const double *n, double *qM, double *qP,
                 double *nflux, double *lambda_out) {
   int m = orc_meqn(ph), rc;
+  if (ph->eqn == ORC_EULER && ph->flux == ORC_FLUX_HLL) return euler_hll(ph, n, qM, qP, nflux, lambda_out);
+  if (ph->eqn == ORC_EULER && ph->flux == ORC_FLUX_ROE) return euler_roe(ph, n, qM, qP, nflux, lambda_out);
+  if (ph->eqn == ORC_EULER && ph->flux == ORC_FLUX_WAVE) {
+    /* "Wave": applyWavePropagationFluxes rotates both traces' momentum in
+     * place (wxeulereqn.cc:1292-1300); its result is then overwritten, since
+     * case 4 of DGnumericalFlux has no break and falls through to LF
+     * (:888-892), which runs on the rotated traces. */
+    rotate_momentum(n, qM);
+    rotate_momentum(n, qP);
+  }
   double fM[8], fP[8], gM[8], gP[8];
   /* flux evaluations first, then primitives (wxeulereqn.cc:1097-1104) */
   if ((rc = orc_flux(ph, 0, qM, fM))) return rc;

@@ -68,11 +68,14 @@ int orc_elem_geom(const orc_ops *ops, const double *x, const double *y, double *
 
 /* Physics configuration of the RHS. */
 enum { ORC_EULER = 0, ORC_PHMAXWELL = 1 };
-enum { ORC_FLUX_LF = 0, ORC_FLUX_UPWIND = 1 };
+/* Euler numerical fluxes follow WxEulerEqn::DGnumericalFlux
+ * (wxeulereqn.cc:873-894): LF, HLL, Roe ("Wave" falls through to LF).
+ * Maxwell: LF, or the build's exact upwind flux (UPWIND). */
+enum { ORC_FLUX_LF = 0, ORC_FLUX_UPWIND = 1, ORC_FLUX_HLL = 2, ORC_FLUX_ROE = 3, ORC_FLUX_WAVE = 4 };
 enum { ORC_BC_NONE = 0, ORC_BC_VORTEX = 1, ORC_BC_TM = 2, ORC_BC_PEC = 3, ORC_BC_COPY = 4 };
 typedef struct {
   int eqn;              /* ORC_EULER (m = 5) or ORC_PHMAXWELL (m = 8)          */
-  int flux;             /* ORC_FLUX_LF, or ORC_FLUX_UPWIND (Maxwell, build ext) */
+  int flux;             /* ORC_FLUX_*: UPWIND for Maxwell (build ext), HLL/ROE/WAVE Euler */
   double gamma, min_dens, min_pres;
   double c0, chi, gph;
   int nbc;

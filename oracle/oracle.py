@@ -16,7 +16,7 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 ORC_EULER, ORC_PHMAXWELL = 0, 1
-ORC_FLUX_LF, ORC_FLUX_UPWIND = 0, 1
+ORC_FLUX_LF, ORC_FLUX_UPWIND, ORC_FLUX_HLL, ORC_FLUX_ROE, ORC_FLUX_WAVE = 0, 1, 2, 3, 4
 ORC_BC_NONE, ORC_BC_VORTEX, ORC_BC_TM, ORC_BC_PEC, ORC_BC_COPY = 0, 1, 2, 3, 4
 
 _dp = ctypes.POINTER(ctypes.c_double)

@@ -44,7 +44,8 @@ def vortex_bc(u0=1.0, v0=0.0, x0=0.0, y0=0.0, gamma=1.4, beta=5.0):
 
 def oracle_phys(params):
     eqn = oracle_eqn(params.eqn)
-    flux = O.ORC_FLUX_UPWIND if params.flux == E.FLUX_UPWIND else O.ORC_FLUX_LF
+    flux = {E.FLUX_LF: O.ORC_FLUX_LF, E.FLUX_UPWIND: O.ORC_FLUX_UPWIND, E.FLUX_HLL: O.ORC_FLUX_HLL,
+            E.FLUX_ROE: O.ORC_FLUX_ROE, E.FLUX_WAVE: O.ORC_FLUX_WAVE}[params.flux]
     return O.phys(eqn, flux=flux, gamma=params.gas_gamma, min_dens=params.min_density,
                   min_pres=params.min_pressure, c0=params.c0, chi=params.chi,
                   gph=params.gamma_ph, bcs=params.bcs)
@@ -118,7 +119,8 @@ def walled_unit_square(n, jitter=0.1, seed=1):
     return E.structured_mesh(n, 0.0, 1.0, 0.0, 1.0, jitter, seed, False)
 
 
-def euler_case(n=16, N=3, m=4, periodic=True, precision=E.PREC_FP64, L=10.0, seed=0):
+def euler_case(n=16, N=3, m=4, periodic=True, precision=E.PREC_FP64, L=10.0, seed=0,
+               flux=E.FLUX_LF):
     eqn = E.EQN_EULER4 if m == 4 else E.EQN_EULER5
     if periodic:
         mesh = periodic_square(n, L=L, seed=seed)
@@ -126,7 +128,7 @@ def euler_case(n=16, N=3, m=4, periodic=True, precision=E.PREC_FP64, L=10.0, see
     else:
         mesh = E.structured_mesh(n, -L / 2, L / 2, -L / 2, L / 2, 0.1, seed, False)
         bcs = [vortex_bc()]
-    return Case(mesh, E.Params(N, eqn, precision=precision, bcs=bcs), "euler")
+    return Case(mesh, E.Params(N, eqn, flux=flux, precision=precision, bcs=bcs), "euler")
 
 
 def maxwell_case(n=12, N=5, m=3, bc=E.BC_MAXWELL_TM, flux=E.FLUX_LF, precision=E.PREC_FP64,

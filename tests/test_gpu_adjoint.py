@@ -140,3 +140,17 @@ def test_autograd_function_against_finite_differences():
         ad = float((g * d).sum())
         print("directional derivative: FD %.12e adjoint %.12e rel %.2e" % (fd, ad, abs(fd - ad) / abs(ad)))
         assert abs(fd - ad) <= 1e-6 * abs(ad)
+
+
+def test_adjoint_rejects_riemann_fluxes():
+    """The discrete adjoint covers LF and the TM upwind flux; HLL, Roe and
+    Wave handles refuse it with APL_ERR_UNSUPPORTED rather than returning a
+    wrong adjoint."""
+    for flux in (E.FLUX_HLL, E.FLUX_ROE, E.FLUX_WAVE):
+        case = euler_case(n=4, N=2, m=4, flux=flux)
+        q = case.state()
+        eng = E.Engine(case.mesh, case.params)
+        with pytest.raises(E.ApolloError) as ei:
+            eng.adjoint_rhs(0.0, q, q.copy(), np.zeros_like(q))
+        assert ei.value.status == 11
+        eng.close()

@@ -81,6 +81,30 @@ def test_tm_upwind_fp64():
     fp64_parity(case, _rhs(case, q), q)
 
 
+@pytest.mark.parametrize("flux", [E.FLUX_HLL, E.FLUX_ROE, E.FLUX_WAVE])
+@pytest.mark.parametrize("m", [4, 5])
+@pytest.mark.parametrize("N", [1, 4, 8])
+def test_euler_riemann_fluxes_fp64(flux, m, N):
+    """Apollo's HLL, Roe and "Wave" fluxes (Roe with its back-rotation as
+    written, Wave as LF of the face-frame traces), periodic and with the
+    isentropic-vortex boundary."""
+    for periodic in (True, False):
+        case = euler_case(n=6 if N == 8 else 8, N=N, m=m, periodic=periodic, flux=flux)
+        q = case.state()
+        fp64_parity(case, _rhs(case, q, 0.21), q, 0.21)
+
+
+@pytest.mark.parametrize("flux", [E.FLUX_HLL, E.FLUX_ROE, E.FLUX_WAVE])
+def test_euler_riemann_fluxes_reduced_precision(flux):
+    case = euler_case(n=8, N=4, m=4, flux=flux)
+    q = case.state()
+    ref, _ = case.oracle_rhs(q)
+    e32 = rel_l2(_rhs(case, q, precision=E.PREC_FP32_MFMA), ref)
+    e3 = rel_l2(_rhs(case, q, precision=E.PREC_BF16X3), ref)
+    print("flux %d: FP32 %.3e BF16x3 %.3e" % (flux, e32, e3))
+    assert e32 <= 2e-5 and e3 <= 1e-3
+
+
 @pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6, 7, 8])
 def test_all_orders_fp64(N):
     for case in (euler_case(n=6, N=N, m=4), maxwell_case(n=5, N=N, m=3)):

@@ -26,10 +26,15 @@ def _frame(case, t0, t1, dt0, cfl, stages=5, q=None, engine_q=None):
     return q0, qg, dts, from_oracle(case.params.eqn, qo), dto
 
 
-@pytest.mark.parametrize("name", ["euler_periodic", "euler_vortex_bc", "maxwell_pec_upwind"])
+@pytest.mark.parametrize("name", ["euler_periodic", "euler_vortex_bc", "maxwell_pec_upwind",
+                                  "euler_hll_vortex_bc", "euler5_hll_periodic"])
 def test_ssp_frame_matches_oracle(name):
     if name == "euler_periodic":
         case, cfl = euler_case(n=8, N=3, m=4), 0.5
+    elif name == "euler_hll_vortex_bc":
+        case, cfl = euler_case(n=8, N=3, m=4, periodic=False, flux=E.FLUX_HLL), 0.5
+    elif name == "euler5_hll_periodic":
+        case, cfl = euler_case(n=8, N=3, m=5, flux=E.FLUX_HLL), 0.5
     elif name == "euler_vortex_bc":
         case, cfl = euler_case(n=8, N=3, m=4, periodic=False), 0.5
     else:
@@ -46,6 +51,24 @@ def test_ssp_frame_matches_oracle(name):
     err = rel_l2(qg - q0, qo - q0)
     print("%s: increment rel L2 vs oracle %.3e" % (name, err))
     assert err <= 1e-11
+
+
+def test_ssp_roe_as_written_loses_positivity():
+    """Apollo's Roe flux puts the y-momentum flux in the x-momentum slot and
+    leaves the y-momentum slot at 0 (wxeulereqn.cc:1259-1260), so its surface
+    term is inconsistent and a frame of the vortex ends in a positivity error
+    in the oracle and in the engine alike."""
+    case = euler_case(n=8, N=3, m=5, flux=E.FLUX_ROE)
+    q0 = case.state()
+    _, ms = case.oracle_rhs(q0)
+    s0 = O.suggested_dt(case.omesh, 0.5, ms)
+    with pytest.raises(ValueError):
+        O.ssp_advance(case.omesh, case.ophys, 0.0, 3 * s0, s0, 0.5, to_oracle(case.params.eqn, q0))
+    eng = E.Engine(case.mesh, case.params)
+    with pytest.raises(E.ApolloError) as ei:
+        eng.ssp_advance(0.0, 3 * s0, s0, 0.5, q0.copy())
+    eng.close()
+    assert ei.value.status in (8, 9)      # APL_ERR_NEG_DENSITY / APL_ERR_NEG_PRESSURE
 
 
 def test_ssp_frame_device_pointer_and_stage_count():

@@ -248,6 +248,151 @@ def test_euler_flux_closed_form():
     assert np.allclose(G, [rho * v, rho * v * u, rho * v * v + p, rho * v * w, (E + p) * v], rtol=1e-15)
 
 
+def _rotated(n, q):
+    out = q.copy()
+    out[1], out[2] = n[0] * q[1] + n[1] * q[2], -n[1] * q[1] + n[0] * q[2]
+    return out
+
+
+def _xflux_2d(q, g=1.4):
+    """x-flux of the 2D Euler equations (rho, rho u, rho v, E), textbook form."""
+    rho, u, v, E = q[0], q[1] / q[0], q[2] / q[0], q[3]
+    p = (g - 1) * (E - 0.5 * rho * (u * u + v * v))
+    return np.array([rho * u, rho * u * u + p, rho * u * v, (E + p) * u])
+
+
+def _roe_avg(a, b, g=1.4):
+    def prim(q):
+        rho, u, v = q[0], q[1] / q[0], q[2] / q[0]
+        p = (g - 1) * (q[-1] - 0.5 * rho * (u * u + v * v))
+        return rho, u, v, (q[-1] + p) / rho, math.sqrt(g * p / rho)
+    ra, ua, va, Ha, ca = prim(a)
+    rb, ub, vb, Hb, cb = prim(b)
+    wa, wb = math.sqrt(ra), math.sqrt(rb)
+    avg = [(wa * x + wb * y) / (wa + wb) for x, y in ((ua, ub), (va, vb), (Ha, Hb))]
+    return avg, (ua, ca), (ub, cb)
+
+
+def _euler_state2d(rng, mach=0.5):
+    rho = 0.6 + rng.random()
+    u, v = rng.standard_normal(2) * mach
+    p = 0.5 + rng.random()
+    return np.array([rho, rho * u, rho * v, 0.0, p / 0.4 + 0.5 * rho * (u * u + v * v)])
+
+
+def test_euler_hll_flux():
+    """HLL (wxeulereqn.cc:1119-1202) against Toro's HLL flux with Einfeldt's
+    speeds, written independently: F = (SR FL - SL FR + SL SR (qR - qL)) /
+    (SR - SL) in the face frame, with F = FL for SL >= 0 and F = FR for
+    SR <= 0; plus consistency and conservation."""
+    ph = O.phys(O.ORC_EULER, flux=O.ORC_FLUX_HLL)
+    rng = np.random.default_rng(11)
+    cases = 0
+    for it in range(60):
+        a, b = _euler_state2d(rng, mach=[0.3, 1.5, 3.0][it % 3]), _euler_state2d(rng)
+        th = rng.random() * 2 * np.pi
+        n = np.array([np.cos(th), np.sin(th)])
+        fs, lam = O.numflux(ph, n, a, b)
+        A, B = _rotated(n, a)[[0, 1, 2, 4]], _rotated(n, b)[[0, 1, 2, 4]]
+        (u, v, H), (uL, cL), (uR, cR) = _roe_avg(A, B)
+        c = math.sqrt(0.4 * (H - 0.5 * (u * u + v * v)))
+        SL, SR = min(uL - cL, u - c), max(uR + cR, u + c)
+        FL, FR = _xflux_2d(A), _xflux_2d(B)
+        if SL >= 0:
+            fx = FL
+        elif SR <= 0:
+            fx = FR
+        else:
+            fx = (SR * FL - SL * FR + SL * SR * (B - A)) / (SR - SL)
+        cases += SL >= 0 or SR <= 0
+        want = np.array([fx[0], n[0] * fx[1] - n[1] * fx[2], n[1] * fx[1] + n[0] * fx[2], 0.0, fx[3]])
+        assert np.abs(fs - want).max() <= 1e-13 * max(1.0, np.abs(want).max())
+        # fastest wave: max over both sides of |u| + c
+        sp = [math.hypot(q[1], q[2]) / q[0] + math.sqrt(1.4 * 0.4 * (q[4] - 0.5 * (q[1] ** 2 + q[2] ** 2) / q[0]) / q[0])
+              for q in (a, b)]
+        assert abs(lam - max(sp)) <= 1e-14 * lam
+        # conservation
+        f2, l2 = O.numflux(ph, -n, b, a)
+        assert np.abs(fs + f2).max() <= 1e-13 * max(1.0, np.abs(fs).max()) and l2 == lam
+    assert cases >= 5          # supersonic branches were exercised
+    # consistency, 3D momentum included: Apollo zeroes the w-momentum flux
+    for _ in range(10):
+        q = _euler_state(rng)
+        th = rng.random() * 2 * np.pi
+        n = np.array([np.cos(th), np.sin(th)])
+        fs, _ = O.numflux(ph, n, q, q)
+        phys = n[0] * O.flux(ph, 0, q) + n[1] * O.flux(ph, 1, q)
+        assert np.abs(fs[[0, 1, 2, 4]] - phys[[0, 1, 2, 4]]).max() < 1e-13 and fs[3] == 0.0
+
+
+def test_euler_roe_flux():
+    """Roe (wxeulereqn.cc:1204-1268).  The face-frame flux is checked against
+    0.5 (FL + FR) - 0.5 |A| (qR - qL), with A the 2D Euler Jacobian at the
+    Roe average and |A| = p(A), p the Lagrange interpolant of |lambda| at the
+    eigenvalues u - c, u, u + c (A is diagonalizable, so this is exact).
+    Apollo's back-rotation writes nflux[1] twice, so the reported fluxes are
+    (mass, y-momentum normal flux in the x-momentum slot, 0, 0, energy)."""
+    ph = O.phys(O.ORC_EULER, flux=O.ORC_FLUX_ROE)
+    rng = np.random.default_rng(12)
+    g1 = 0.4
+    for it in range(60):
+        a, b = _euler_state2d(rng, mach=[0.3, 1.5][it % 2]), _euler_state2d(rng)
+        th = rng.random() * 2 * np.pi
+        n = np.array([np.cos(th), np.sin(th)])
+        fs, lam = O.numflux(ph, n, a, b)
+        A, B = _rotated(n, a)[[0, 1, 2, 4]], _rotated(n, b)[[0, 1, 2, 4]]
+        (u, v, H), _, _ = _roe_avg(A, B)
+        k = 0.5 * (u * u + v * v)
+        c = math.sqrt(g1 * (H - k))
+        J = np.array([[0, 1, 0, 0],
+                      [g1 * k - u * u, (3 - 1.4) * u, -g1 * v, g1],
+                      [-u * v, v, u, 0],
+                      [u * (g1 * k - H), H - g1 * u * u, -g1 * u * v, 1.4 * u]])
+        lams = (u - c, u, u + c)
+        absJ = np.zeros((4, 4))
+        for i, li in enumerate(lams):
+            term = abs(li) * np.eye(4)
+            for j, lj in enumerate(lams):
+                if j != i:
+                    term = term @ (J - lj * np.eye(4)) / (li - lj)
+            absJ += term
+        fx = 0.5 * (_xflux_2d(A) + _xflux_2d(B)) - 0.5 * absJ @ (B - A)
+        want = np.array([fx[0], n[1] * fx[1] + n[0] * fx[2], 0.0, 0.0, fx[3]])
+        assert np.abs(fs - want).max() <= 1e-12 * max(1.0, np.abs(want).max()), (it, fs, want)
+        f2, l2 = O.numflux(ph, -n, b, a)
+        assert l2 == lam and abs(fs[0] + f2[0]) <= 1e-13 * max(1.0, abs(fs[0]))
+    # consistency of the slots Apollo keeps
+    for _ in range(10):
+        q = _euler_state(rng)
+        th = rng.random() * 2 * np.pi
+        n = np.array([np.cos(th), np.sin(th)])
+        fs, _ = O.numflux(ph, n, q, q)
+        phys = n[0] * O.flux(ph, 0, q) + n[1] * O.flux(ph, 1, q)
+        assert np.abs(fs[[0, 1, 4]] - phys[[0, 2, 4]]).max() < 1e-13
+        assert fs[2] == 0.0 and fs[3] == 0.0
+
+
+def test_euler_wave_flux_falls_through_to_lf():
+    """Apollo's "Wave" choice (wxeulereqn.cc:888-892, 1283-1300): the
+    wave-propagation routine rotates both traces' momentum to the face frame
+    in place, its own result is overwritten by the LF case it falls into, so
+    the flux is LF of the rotated traces.  Along n = (1, 0) the rotation is
+    the identity and Wave equals LF bit for bit; along n = (0, 1) it maps
+    (rho u, rho v) to (rho v, -rho u)."""
+    pw, pl = O.phys(O.ORC_EULER, flux=O.ORC_FLUX_WAVE), O.phys(O.ORC_EULER)
+    rng = np.random.default_rng(13)
+    for _ in range(20):
+        a, b = _euler_state(rng), _euler_state(rng)
+        ex = np.array([1.0, 0.0])
+        assert np.array_equal(O.numflux(pw, ex, a, b)[0], O.numflux(pl, ex, a, b)[0])
+        ey = np.array([0.0, 1.0])
+        ra, rb = a.copy(), b.copy()
+        ra[1], ra[2], rb[1], rb[2] = a[2], -a[1], b[2], -b[1]
+        fw, lw = O.numflux(pw, ey, a, b)
+        fl, ll = O.numflux(pl, ey, ra, rb)
+        assert np.array_equal(fw, fl) and lw == ll
+
+
 def test_maxwell_tm_closure():
     """TM fields (Ez, Bx, By) with chi = gamma = 0 never feed Ex, Ey, Bz, phi, psi."""
     ph = O.phys(O.ORC_PHMAXWELL, c0=1.0)
