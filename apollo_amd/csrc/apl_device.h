@@ -369,6 +369,84 @@ __device__ __forceinline__ unsigned num_flux(const PhysParams &P, double nx, dou
   return ef;
 }
 
+// Apollo's 5-variable Euler state (rho, rho u, rho v, rho w, E) from the
+// engine's layout: EULER4 (rho, rho u, rho v, E) embeds with rho w = 0.
+template <int EQN>
+__device__ __forceinline__ void load_euler5(const double *q, double *q5) {
+  if constexpr (EQN == APL_EQN_EULER5) {
+#pragma unroll
+    for (int c = 0; c < 5; ++c) q5[c] = q[c];
+  } else {
+    q5[0] = q[0];
+    q5[1] = q[1];
+    q5[2] = q[2];
+    q5[3] = 0.0;
+    q5[4] = q[3];
+  }
+}
+
+template <int EQN>
+__device__ __forceinline__ void store_euler5(const double *q5, double *q) {
+  if constexpr (EQN == APL_EQN_EULER5) {
+#pragma unroll
+    for (int c = 0; c < 5; ++c) q[c] = q5[c];
+  } else {
+    q[0] = q5[0];
+    q[1] = q5[1];
+    q[2] = q5[2];
+    q[3] = q5[4];
+  }
+}
+
+// Euler exterior states on the 5-variable state: isentropic vortex
+// (wxisentropicvortexbc.cc:20-43), wall (wxeulerwallbc.cc:18-25), inflow
+// (wxeulerinflowbc.cc:21-36), Dirichlet (wxeulerdirichletbc.cc:22-30); any
+// other kind copies (zero gradient, wxeulerzerogradientbc.cc:18-25).
+__device__ __forceinline__ void euler_bc5(int kind, const double *p, double t, double x, double y,
+                                          double nx, double ny, const double *q, double *o) {
+#pragma unroll
+  for (int c = 0; c < 5; ++c) o[c] = q[c];
+  const double pi = 3.141592653589793;
+  if (kind == APL_BC_ISENTROPIC_VORTEX) {
+    const double uo = p[0], vo = p[1], xo = p[2], yo = p[3], gamma = p[4], beta = p[5];
+    const double xmut = x - uo * t, ymvt = y - vo * t;
+    const double r = sqrt((xmut - xo) * (xmut - xo) + (ymvt - yo) * (ymvt - yo));
+    const double ex = exp(1. - r * r);
+    const double u = uo - beta * ex * (ymvt - yo) / (2. * pi);
+    const double v = vo + beta * ex * (xmut - xo) / (2. * pi);
+    const double base = 1. - ((gamma - 1.) * beta * beta * exp(2. * (1. - r * r)) / (16. * gamma * pi * pi));
+    const double rho1 = pow(base, 1. / (gamma - 1));
+    const double p1 = pow(rho1, gamma);
+    o[0] = rho1;
+    o[1] = rho1 * u;
+    o[2] = rho1 * v;
+    o[3] = 0.0;
+    o[4] = p1 / (gamma - 1) + 0.5 * rho1 * (u * u + v * v);
+  } else if (kind == APL_BC_EULER_WALL) {
+    const double qn = nx * q[1] + ny * q[2];
+    o[1] = q[1] - 2. * qn * nx;
+    o[2] = q[2] - 2. * qn * ny;
+    o[3] = 0.0;
+  } else if (kind == APL_BC_EULER_INFLOW) {
+    const double mach = p[0], angle = p[1], gamma = p[2], pres = p[3], rho = p[4];
+    const double sound = sqrt(gamma * pres / rho);
+    const double u = mach * sound * cos(angle * pi / 180.);
+    const double v = mach * sound * sin(angle * pi / 180.);
+    o[0] = rho;
+    o[1] = rho * u;
+    o[2] = rho * v;
+    o[3] = 0.0;
+    o[4] = pres / (gamma - 1) + 0.5 * rho * (u * u + v * v);
+  } else if (kind == APL_BC_EULER_DIRICHLET) {
+    const double rho = p[0], u = p[1], v = p[2], w = p[3], gamma = p[4], pres = p[5];
+    o[0] = rho;
+    o[1] = rho * u;
+    o[2] = rho * v;
+    o[3] = rho * w;
+    o[4] = pres / (gamma - 1) + 0.5 * rho * (u * u + v * v + w * w);
+  }
+}
+
 // Exterior state at a boundary Gauss point (wxgridbc.cc:21-26 and
 // wxisentropicvortexbc.cc:20-43, wxmaxwelltranversemagneticbc.cc:13-25,
 // wxphmaxwellconductingwallbc.cc:12-41).
@@ -382,26 +460,11 @@ __device__ __forceinline__ void boundary_state(const BCTable &B, int idx, double
 #pragma unroll
   for (int c = 0; c < M; ++c) qP[c] = qM[c];
   if constexpr (EQN == APL_EQN_EULER5 || EQN == APL_EQN_EULER4) {
-    if (kind == APL_BC_ISENTROPIC_VORTEX) {
-      const double uo = p[0], vo = p[1], xo = p[2], yo = p[3], gamma = p[4], beta = p[5];
-      const double pi = 3.141592653589793;
-      const double xmut = x - uo * t, ymvt = y - vo * t;
-      const double r = sqrt((xmut - xo) * (xmut - xo) + (ymvt - yo) * (ymvt - yo));
-      const double ex = exp(1. - r * r);
-      const double u = uo - beta * ex * (ymvt - yo) / (2. * pi);
-      const double v = vo + beta * ex * (xmut - xo) / (2. * pi);
-      const double base = 1. - ((gamma - 1.) * beta * beta * exp(2. * (1. - r * r)) / (16. * gamma * pi * pi));
-      const double rho1 = pow(base, 1. / (gamma - 1));
-      const double p1 = pow(rho1, gamma);
-      qP[0] = rho1;
-      qP[1] = rho1 * u;
-      qP[2] = rho1 * v;
-      if constexpr (EQN == APL_EQN_EULER5) {
-        qP[3] = 0.0;
-        qP[4] = p1 / (gamma - 1) + 0.5 * rho1 * (u * u + v * v);
-      } else {
-        qP[3] = p1 / (gamma - 1) + 0.5 * rho1 * (u * u + v * v);
-      }
+    if (kind != APL_BC_COPY && kind != APL_BC_NONE) {
+      double q5[5], o5[5];
+      load_euler5<EQN>(qM, q5);
+      euler_bc5(kind, p, t, x, y, nx, ny, q5, o5);
+      store_euler5<EQN>(o5, qP);
     }
   } else if constexpr (EQN == APL_EQN_PHMAXWELL8) {
     if (kind == APL_BC_MAXWELL_TM) {

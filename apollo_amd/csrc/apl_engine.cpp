@@ -15,6 +15,7 @@
 #include "../../include/apollo_dg.h"
 #include "apl_adjoint.h"
 #include "apl_device.h"
+#include "apl_limiter.h"
 #include "apl_host.h"
 
 namespace apl {
@@ -65,6 +66,8 @@ struct apl_handle_s {
   double *d_ckpt = nullptr;
   long long ckpt_slots = 0;
   double *d_opart = nullptr;
+  // limiter: cell averages, gradients, weight and node tables
+  double *d_lavg = nullptr, *d_lgrad = nullptr, *d_ltab = nullptr;
 };
 
 // ---------------------------------------------------------------------------
@@ -387,7 +390,7 @@ static void free_handle(apl_handle *h) {
   void *bufs[] = {h->d_geom, h->d_vert, h->d_I1D, h->d_fcode, h->d_fmask, h->d_A1, h->d_A2,
                   h->d_A3, h->d_err, h->d_speed, h->d_t, h->d_qa, h->d_qb, h->d_res, h->d_out,
                   h->d_ckpt, h->d_Vc, h->d_cub, h->d_sbar, h->d_qbar, h->d_rbar, h->d_stg,
-                  h->d_seg, h->d_opart};
+                  h->d_seg, h->d_opart, h->d_lavg, h->d_lgrad, h->d_ltab};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (double *b : h->d_adjops)
@@ -468,6 +471,22 @@ int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out)
   if (params->eqn == APL_EQN_MAXWELL_TM3 && (params->chi != 0.0 || params->gamma_ph != 0.0))
     return fail(nullptr, APL_ERR_ARG, "MAXWELL_TM3 requires chi == gamma == 0");
   if (params->n_bc < 0 || params->n_bc > APL_MAX_BC) return fail(nullptr, APL_ERR_ARG, "n_bc");
+  for (int i = 0; i < params->n_bc; ++i)
+    if (params->eqn == APL_EQN_EULER4 && params->bc[i].kind == APL_BC_EULER_DIRICHLET &&
+        params->bc[i].p[3] != 0.0)
+      return fail(nullptr, APL_ERR_ARG, "EULER4 has no z momentum: Dirichlet zvelocity must be 0");
+  if (params->limiter) {
+    if (params->limiter != 1 || !euler)
+      return fail(nullptr, APL_ERR_ARG, "limiter: the Hesthaven-Warburton limiter is for the Euler equations");
+    if (params->order > 4)
+      return fail(nullptr, APL_ERR_UNSUPPORTED,
+                  "limiter: orders above 4 read outside the face-average array in the reference");
+    for (int i = 0; i < params->n_bc; ++i)
+      if (params->bc[i].kind == APL_BC_ISENTROPIC_VORTEX)
+        return fail(nullptr, APL_ERR_UNSUPPORTED,
+                    "limiter: the isentropic-vortex state reads three coordinates from the "
+                    "limiter's two-entry array in the reference");
+  }
   if (mesh->K < 1 || mesh->Nv < 3 || !mesh->VX || !mesh->VY || !mesh->EToV)
     return fail(nullptr, APL_ERR_ARG, "empty or incomplete mesh");
   for (long i = 0; i < 3L * mesh->K; ++i)
@@ -580,8 +599,77 @@ int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out)
   if ((rc = dev_alloc(h, (void **)&h->d_err, sizeof(unsigned) * 4))) return bail(rc);
   if ((rc = dev_alloc(h, (void **)&h->d_speed, sizeof(double) * ((h->K + 15) / 16)))) return bail(rc);
   if ((rc = dev_alloc(h, (void **)&h->d_t, sizeof(double) * 2))) return bail(rc);
+  if (params->limiter) {
+    // ave[Np] = 0.5 sum_m Mass[m][k], Mass = (V V^T)^-1 (wxnodaldggeometry2d.h:84-98),
+    // then r, s and drop[i][j] = delta_ij - ave[j]
+    std::vector<double> mass((size_t)Np * Np), tab((size_t)3 * Np + (size_t)Np * Np);
+    if (!invert(Np, o.VVT.data(), mass.data())) {
+      h->last_error = "mass matrix inversion failed";
+      return bail(APL_ERR_ORDER);
+    }
+    for (int k = 0; k < Np; ++k) {
+      double a = 0.0;
+      for (int m = 0; m < Np; ++m) a += 0.5 * mass[(size_t)k + (size_t)m * Np];
+      tab[k] = a;
+      tab[Np + k] = o.r[k];
+      tab[2 * Np + k] = o.s[k];
+    }
+    for (int i = 0; i < Np; ++i)
+      for (int j = 0; j < Np; ++j) tab[3 * Np + (size_t)i * Np + j] = -tab[j] + (i == j ? 1. : 0.);
+    if ((rc = dev_upload<double>(h, &h->d_ltab, tab.data(), tab.size()))) return bail(rc);
+    if ((rc = dev_alloc(h, (void **)&h->d_lavg, sizeof(double) * 5 * (size_t)h->K))) return bail(rc);
+    if ((rc = dev_alloc(h, (void **)&h->d_lgrad, sizeof(double) * 40 * (size_t)h->K))) return bail(rc);
+  }
   *out = h;
   return APL_OK;
+}
+
+// The limiter's three passes on the handle's stream (qout may equal qin).
+static int enqueue_limit(apl_handle *h, const double *qin, double *qout) {
+  LimArgs a;
+  a.K = h->K;
+  a.qin = qin;
+  a.qout = qout;
+  a.avg = h->d_lavg;
+  a.grad = h->d_lgrad;
+  a.geom = h->d_geom;
+  a.vert = h->d_vert;
+  a.fcode = h->d_fcode;
+  a.fmask = h->d_fmask;
+  a.tab = h->d_ltab;
+  a.gamma = h->params.lim_gas_gamma;
+  a.min_dens = h->params.lim_min_density;
+  a.min_pres = h->params.lim_min_pressure;
+  a.err = h->d_err;
+  a.bc = base_args(h).bc;
+  int rc;
+  switch (h->params.order) {
+    case 1: rc = launch_limiter<1>(h->params.eqn, h->stream, a); break;
+    case 2: rc = launch_limiter<2>(h->params.eqn, h->stream, a); break;
+    case 3: rc = launch_limiter<3>(h->params.eqn, h->stream, a); break;
+    case 4: rc = launch_limiter<4>(h->params.eqn, h->stream, a); break;
+    default: rc = APL_ERR_UNSUPPORTED;
+  }
+  if (rc) return fail(h, rc, "limiter launch");
+  return APL_OK;
+}
+
+int apl_limit(apl_handle *h, const double *q_in, double *q_out, int where) {
+  if (!h || !q_in || !q_out) return APL_ERR_ARG;
+  if (!h->params.limiter) return fail(h, APL_ERR_ARG, "apl_limit: handle created without the limiter");
+  int rc;
+  if ((rc = ensure_work(h))) return rc;
+  HIPCHK(h, hipMemsetAsync(h->d_err, 0, sizeof(unsigned), h->stream));
+  if (where == APL_HOST) {
+    const size_t bytes = sizeof(double) * state_len(h);
+    HIPCHK(h, hipMemcpyAsync(h->d_qa, q_in, bytes, hipMemcpyHostToDevice, h->stream));
+    if ((rc = enqueue_limit(h, h->d_qa, h->d_qa))) return rc;
+    HIPCHK(h, hipMemcpyAsync(q_out, h->d_qa, bytes, hipMemcpyDeviceToHost, h->stream));
+  } else {
+    if ((rc = enqueue_limit(h, q_in, q_out))) return rc;
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return check_flags(h);
 }
 
 int apl_destroy(apl_handle *h) {
@@ -780,7 +868,9 @@ static int enqueue_step(apl_handle *h, double *src, double *dst, double dt, bool
     a.rk_a = rk4a[s];
     a.rk_b = rk4b[s];
     a.tofs = rk4c[s] * dt;
-    int rc = dispatch_rhs(h, a);
+    int rc;
+    if (h->params.limiter && (rc = enqueue_limit(h, bufs[s & 1], bufs[s & 1]))) return rc;
+    rc = dispatch_rhs(h, a);
     if (rc) return rc;
   }
   if (advance) {
@@ -889,8 +979,16 @@ int apl_ssp_advance(apl_handle *h, double t0, double t_end, double dt0, double c
     double dt = dt_temp;
     if (t + dt > t_end) dt = t_end - t;                                      // MATCHSTEP
     if (n >= max_steps) return fail(h, APL_ERR_ARG, "apl_ssp_advance: more than max_steps steps");
+    // TSSSP works on a copy w of sol; with the limiter, step() limits w in
+    // place before each RHS (wxnodaldg2dmethod.cc:243-247), so the first
+    // stage limits sol into the free buffer and later stages in place
     const double *src = sol;
     for (int i = 0; i < stages - 1; ++i) {
+      if (h->params.limiter) {
+        double *lim = (i == 0) ? bufs[1] : bufs[(i - 1) & 1];
+        if ((rc = enqueue_limit(h, src, lim))) return rc;
+        src = lim;
+      }
       a.q = src;
       a.out = bufs[i & 1];
       a.z = nullptr;
@@ -901,6 +999,11 @@ int apl_ssp_advance(apl_handle *h, double t0, double t_end, double dt0, double c
       a.speed_part = nullptr;
       if ((rc = dispatch_rhs(h, a))) return rc;
       src = bufs[i & 1];
+    }
+    if (h->params.limiter) {
+      // with two stages the first limited copy sits in bufs[1] and the
+      // stage result in bufs[0]; src is always a work buffer here
+      if ((rc = enqueue_limit(h, src, (double *)src))) return rc;
     }
     a.q = src;
     a.out = sol;
@@ -1059,6 +1162,7 @@ static int dispatch_adjoint(apl_handle *h, const AdjArgs &a) {
 int apl_adjoint_rhs(apl_handle *h, double t, const double *q, const double *lam, double *out,
                     int where) {
   if (!h || !q || !lam || !out) return APL_ERR_ARG;
+  if (h->params.limiter) return fail(h, APL_ERR_UNSUPPORTED, "the adjoint does not cover the limiter");
   int rc;
   if ((rc = ensure_adjoint_ops(h)) || (rc = ensure_work(h))) return rc;
   const size_t bytes = sizeof(double) * state_len(h);
@@ -1113,6 +1217,7 @@ int apl_lserk4_adjoint(apl_handle *h, double t0, double dt, int nsteps, int ckpt
   if (!h || !q0 || !qbarT || !dJdq0 || nsteps < 0 || ckpt_every < 1) return APL_ERR_ARG;
   if (h->params.precision != APL_PREC_FP64)
     return fail(h, APL_ERR_UNSUPPORTED, "the adjoint runs on FP64 handles");
+  if (h->params.limiter) return fail(h, APL_ERR_UNSUPPORTED, "the adjoint does not cover the limiter");
   int rc;
   if ((rc = ensure_adjoint_ops(h)) || (rc = ensure_work(h))) return rc;
   const size_t n = state_len(h), bytes = sizeof(double) * n;

@@ -3,6 +3,7 @@
 // orders build in parallel.
 #include "apl_adjoint.h"
 #include "apl_device.h"
+#include "apl_limiter.h"
 
 #ifndef APL_ORDER
 #error "compile with -DAPL_ORDER=<N>"
@@ -114,6 +115,30 @@ int launch_objective(int m, hipStream_t s, const ObjArgs &a, int nblocks) {
 
 template int launch_adjoint<APL_ORDER>(int, int, hipStream_t, const AdjArgs &, int);
 template int launch_objective<APL_ORDER>(int, hipStream_t, const ObjArgs &, int);
+
+template <int N, int EQN>
+static int launch_limiter_typed(hipStream_t s, const LimArgs &a) {
+  const dim3 grid((a.K + 255) / 256), block(256);
+  hipLaunchKernelGGL((lim_average_kernel<N, EQN>), grid, block, 0, s, a);
+  hipLaunchKernelGGL((lim_gradient_kernel<N, EQN>), grid, block, 0, s, a);
+  hipLaunchKernelGGL((lim_reconstruct_kernel<N, EQN>), grid, block, 0, s, a);
+  return hipGetLastError() == hipSuccess ? APL_OK : APL_ERR_HIP;
+}
+
+template <int N>
+int launch_limiter(int eqn, hipStream_t s, const LimArgs &a) {
+  if constexpr (N > 4) {
+    return APL_ERR_UNSUPPORTED;
+  } else {
+    switch (eqn) {
+      case APL_EQN_EULER5: return launch_limiter_typed<N, APL_EQN_EULER5>(s, a);
+      case APL_EQN_EULER4: return launch_limiter_typed<N, APL_EQN_EULER4>(s, a);
+      default: return APL_ERR_UNSUPPORTED;
+    }
+  }
+}
+
+template int launch_limiter<APL_ORDER>(int, hipStream_t, const LimArgs &);
 
 #if APL_ORDER == 1
 __global__ void advance_time_kernel(double *t, double dt) {

@@ -23,6 +23,7 @@ FLUX_LF, FLUX_UPWIND, FLUX_HLL, FLUX_ROE, FLUX_WAVE = 0, 1, 2, 3, 4
 PREC_FP64, PREC_FP32_MFMA, PREC_BF16X3 = 0, 1, 2
 ALGO_AUTO, ALGO_CUBATURE = 0, 1
 BC_NONE, BC_ISENTROPIC_VORTEX, BC_MAXWELL_TM, BC_PEC, BC_COPY = 0, 1, 2, 3, 4
+BC_EULER_WALL, BC_EULER_INFLOW, BC_EULER_DIRICHLET = 5, 6, 7
 HOST, DEVICE = 0, 1
 MAX_BC = 8
 NVARS = {EQN_EULER5: 5, EQN_EULER4: 4, EQN_PHMAXWELL8: 8, EQN_MAXWELL_TM3: 3}
@@ -52,7 +53,9 @@ class _Params(ctypes.Structure):
                 ("gas_gamma", ctypes.c_double),
                 ("min_density", ctypes.c_double), ("min_pressure", ctypes.c_double),
                 ("c0", ctypes.c_double), ("chi", ctypes.c_double), ("gamma_ph", ctypes.c_double),
-                ("n_bc", ctypes.c_int), ("bc", _BC * MAX_BC)]
+                ("n_bc", ctypes.c_int), ("bc", _BC * MAX_BC),
+                ("limiter", ctypes.c_int), ("lim_gas_gamma", ctypes.c_double),
+                ("lim_min_density", ctypes.c_double), ("lim_min_pressure", ctypes.c_double)]
 
 
 class _Info(ctypes.Structure):
@@ -65,7 +68,7 @@ class _Info(ctypes.Structure):
 EXPORTS = [
     "apl_create", "apl_destroy", "apl_get_info", "apl_set_stream", "apl_get_index_maps",
     "apl_get_geometry", "apl_get_operator", "apl_rhs", "apl_initial_dt", "apl_lserk4",
-    "apl_ssp_advance", "apl_adjoint_rhs", "apl_lserk4_adjoint", "apl_objective_disc_sq",
+    "apl_ssp_advance", "apl_limit", "apl_adjoint_rhs", "apl_lserk4_adjoint", "apl_objective_disc_sq",
     "apl_mesh_structured",
     "apl_status_string", "apl_last_error", "apl_host_connectivity", "apl_host_operator",
 ]
@@ -103,6 +106,7 @@ def lib():
     L.apl_ssp_advance.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_double, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int,
                                   ctypes.POINTER(ctypes.c_int), _dp]
+    L.apl_limit.argtypes = [vp, vp, vp, ctypes.c_int]
     L.apl_adjoint_rhs.argtypes = [vp, ctypes.c_double, vp, vp, vp, ctypes.c_int]
     L.apl_lserk4_adjoint.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                      ctypes.c_int, vp, vp, vp, vp, ctypes.c_int]
@@ -209,8 +213,11 @@ class Params:
 
     def __init__(self, order, eqn, flux=FLUX_LF, precision=PREC_FP64, gas_gamma=1.4,
                  min_density=0.0, min_pressure=0.0, c0=1.0, chi=0.0, gamma_ph=0.0, bcs=(),
-                 algorithm=ALGO_AUTO):
+                 algorithm=ALGO_AUTO, limiter=None):
+        """limiter: None, or (gas_gamma, min_density, min_pressure) of an
+        eulerLimiterHW block."""
         self.order, self.eqn, self.flux, self.precision = order, eqn, flux, precision
+        self.limiter = None if limiter is None else tuple(limiter)
         self.algorithm = algorithm
         self.gas_gamma, self.min_density, self.min_pressure = gas_gamma, min_density, min_pressure
         self.c0, self.chi, self.gamma_ph = c0, chi, gamma_ph
@@ -222,6 +229,9 @@ class Params:
         p.algorithm = self.algorithm
         p.gas_gamma, p.min_density, p.min_pressure = self.gas_gamma, self.min_density, self.min_pressure
         p.c0, p.chi, p.gamma_ph = self.c0, self.chi, self.gamma_ph
+        if self.limiter is not None:
+            p.limiter = 1
+            p.lim_gas_gamma, p.lim_min_density, p.lim_min_pressure = self.limiter
         p.n_bc = len(self.bcs)
         for i, (kind, vals) in enumerate(self.bcs):
             p.bc[i].kind = kind
@@ -333,6 +343,14 @@ class Engine:
         self._check(lib().apl_ssp_advance(self._h, t0, t_end, dt0, cfl, stages, qp, where,
                                           max_steps, ctypes.byref(n), _d(dts)), "ssp_advance")
         return dts[:n.value].copy()
+
+    def limit(self, q_in, q_out):
+        """Apollo's Hesthaven-Warburton Euler limiter: q_out = limited q_in
+        (q_out may be q_in)."""
+        ip, where = self._ptr(q_in)
+        op, w2 = self._ptr(q_out)
+        assert where == w2
+        self._check(lib().apl_limit(self._h, ip, op, where), "limit")
 
     def adjoint_rhs(self, t, q, lam, out):
         qp, where = self._ptr(q)

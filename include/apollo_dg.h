@@ -97,7 +97,16 @@ enum apl_bc_kind {
   APL_BC_MAXWELL_TM = 2,        /* Ez -> -Ez, rest copied,
                                    wxmaxwelltranversemagneticbc.cc:13-25     */
   APL_BC_PEC = 3,               /* conducting wall, wxphmaxwellconductingwallbc.cc:12-41 */
-  APL_BC_COPY = 4               /* q+ = q- (zero-gradient outflow)           */
+  APL_BC_COPY = 4,              /* q+ = q- (zero-gradient outflow,
+                                   wxeulerzerogradientbc.cc:18-25)           */
+  APL_BC_EULER_WALL = 5,        /* reflect the normal momentum, rho w = 0,
+                                   wxeulerwallbc.cc:18-25                    */
+  APL_BC_EULER_INFLOW = 6,      /* fixed supersonic inflow, wxeulerinflowbc.cc:21-36
+                                   p[0..4] = Mach_number, Flow_angle (deg),
+                                   gas_gamma, Pressure, density              */
+  APL_BC_EULER_DIRICHLET = 7    /* fixed state, wxeulerdirichletbc.cc:22-30
+                                   p[0..5] = density, xvelocity, yvelocity,
+                                   zvelocity, gas_gamma, pressure            */
 };
 
 enum apl_where { APL_HOST = 0, APL_DEVICE = 1 };
@@ -138,6 +147,14 @@ typedef struct {
   int n_bc;              /* boundary-condition list; a face with tag T uses
                             bc[T-1] (src/subsolvers/wxnodaldg2dmethod.cc:569)  */
   apl_bc bc[APL_MAX_BC];
+  int limiter;           /* 1: Apollo's `Limiter = [eulerLimiterHW]`
+                            (wxnodaldg2dmethod.cc:126-134): the steppers limit
+                            every RHS input first (:243-247); Euler, N <= 4,
+                            no isentropic-vortex boundary condition           */
+  double lim_gas_gamma, lim_min_density, lim_min_pressure;
+                         /* the limiter block's gas_gamma, min_density,
+                            min_pressure (wxhesthavenwarburtoneulerlimiter.cc:
+                            21-23)                                            */
 } apl_params;
 
 typedef struct {
@@ -197,6 +214,16 @@ int apl_get_operator(const apl_handle *h, int which, double *out, int capacity);
 int apl_rhs(apl_handle *h, double t, const double *q, double *rhs,
             double cfl, double *suggested_dt, int where);
 
+/* Replaces WxNodalDG2dMethod::applyLimiter -> WxHestavenWarburtonEulerLimiter::
+ * applyLimiter (wxnodaldg2dmethod.cc:575-582, src/hyperapps/euler/
+ * wxhesthavenwarburtoneulerlimiter.cc:61-484): every element is replaced
+ * by the reconstruction from its cell average and its limited gradient.
+ * q_out may equal q_in.  Needs a handle created with limiter = 1.  apl_rhs
+ * does not limit; Apollo's step() limits its input before the RHS, which is
+ * what apl_lserk4 and apl_ssp_advance do on such handles.  A NaN or Inf in
+ * the result is APL_ERR_NAN (wxnodaldg2dmethod.cc:246). */
+int apl_limit(apl_handle *h, const double *q_in, double *q_out, int where);
+
 /* Initial-condition dt of WxNodalDG2dMethod::init (wxnodaldg2dmethod.cc:
  * 201-228): Roe wave speeds at every node, (2/3) cfl dtscale rmin / maxSpeed. */
 int apl_initial_dt(apl_handle *h, const double *q, double cfl, double *dt, int where);
@@ -205,7 +232,9 @@ int apl_initial_dt(apl_handle *h, const double *q, double cfl, double *dt, int w
  * fixed dt, starting at time t0; q is advanced in place.  Each step runs as
  * one captured hipGraph of five fused RHS + stage-update kernels.  Stands in
  * for the PETSc TS loop around ApSolver::ComputeRHSforTS
- * (src/solvers/apsolver.cc:442-480, src/subsolvers/wxpetsctimestepping.h). */
+ * (src/solvers/apsolver.cc:442-480, src/subsolvers/wxpetsctimestepping.h).
+ * On a handle with the limiter, each stage limits its input state in place
+ * before the RHS. */
 int apl_lserk4(apl_handle *h, double t0, double dt, int nsteps, double *q, int where);
 
 /* Apollo's own stepping of one output frame [t0, t_end], in place of the
@@ -217,7 +246,9 @@ int apl_lserk4(apl_handle *h, double t0, double dt, int nsteps, double *q, int w
  * after each RHS (apsolver.cc:442-480), with the suggested dt of
  * WxNodalDG2dMethod::step at `cfl`.  q is advanced in place; *nsteps
  * receives the step count and dts[0 .. *nsteps) (optional, max_steps long)
- * each step's dt.  More than max_steps steps is APL_ERR_ARG. */
+ * each step's dt.  More than max_steps steps is APL_ERR_ARG.  On a handle
+ * with the limiter, each RHS input (the TSSSP stage vector, never q itself)
+ * is limited in place first, as step() does (wxnodaldg2dmethod.cc:243-247). */
 int apl_ssp_advance(apl_handle *h, double t0, double t_end, double dt0, double cfl, int stages,
                     double *q, int where, int max_steps, int *nsteps, double *dts);
 

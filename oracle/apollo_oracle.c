@@ -692,7 +692,38 @@ void orc_bc(const orc_phys *ph, int bcnum, const double *xc, const double *nx,
       qBC[7] = psi;
       break;
     }
-    default: /* copy */
+    case ORC_BC_EULER_WALL: {                 /* wxeulerwallbc.cc:18-25 */
+      double qn = nx[0] * q[1] + nx[1] * q[2];
+      qBC[0] = q[0];
+      qBC[1] = q[1] - 2. * qn * nx[0];
+      qBC[2] = q[2] - 2. * qn * nx[1];
+      qBC[3] = 0.0;
+      qBC[4] = q[4];
+      break;
+    }
+    case ORC_BC_EULER_INFLOW: {               /* wxeulerinflowbc.cc:21-36 */
+      double mach = P[0], angle = P[1], gamma = P[2], pres = P[3], rho = P[4];
+      double pi = 3.141592653589793;
+      double sound = sqrt(gamma * pres / rho);
+      double u = mach * sound * cos(angle * pi / 180.);
+      double v = mach * sound * sin(angle * pi / 180.);
+      qBC[0] = rho;
+      qBC[1] = rho * u;
+      qBC[2] = rho * v;
+      qBC[3] = 0.0;
+      qBC[4] = pres / (gamma - 1) + 0.5 * rho * (u * u + v * v);
+      break;
+    }
+    case ORC_BC_EULER_DIRICHLET: {            /* wxeulerdirichletbc.cc:22-30 */
+      double rho = P[0], u = P[1], v = P[2], w = P[3], gamma = P[4], pres = P[5];
+      qBC[0] = rho;
+      qBC[1] = rho * u;
+      qBC[2] = rho * v;
+      qBC[3] = rho * w;
+      qBC[4] = pres / (gamma - 1) + 0.5 * rho * (u * u + v * v + w * w);
+      break;
+    }
+    default: /* copy: eulerZeroGradientBC (wxeulerzerogradientbc.cc:18-25) */
       for (int c = 0; c < m; c++) qBC[c] = q[c];
   }
 }
@@ -928,6 +959,7 @@ int orc_lserk4(const orc_ops *o, const orc_phys *ph, int K, const double *x,
     double t = t0 + step * dt;
     for (int s = 0; s < 5 && !rc; s++) {
       double ms;
+      if (ph->limiter && (rc = orc_limit(o, ph, K, x, y, etetf, q, q))) break;
       rc = orc_rhs(o, ph, K, x, y, etetf, t + orc_rk4c[s] * dt, q, rhs, &ms, nthreads);
       for (size_t i = 0; i < n; i++) {
         res[i] = orc_rk4a[s] * res[i] + dt * rhs[i];
@@ -966,11 +998,15 @@ int orc_ssp_advance(const orc_ops *o, const orc_phys *ph, int K, const double *x
     if (steps >= max_steps) { rc = 1; break; }
     double ms = 0.0;
     memcpy(w, q, n * sizeof(double));
+    /* with a limiter, step() limits the vector TSSSP hands it in place
+     * (wxnodaldg2dmethod.cc:243-247): the stage vector w, never q */
     for (int i = 0; i < stages - 1 && !rc; i++) {
+      if (ph->limiter && (rc = orc_limit(o, ph, K, x, y, etetf, w, w))) break;
       rc = orc_rhs(o, ph, K, x, y, etetf, t + dt * (i / (stages - 1.)), w, F, &ms, nthreads);
       for (size_t k = 0; k < n; k++) w[k] = w[k] + dt / (stages - 1.) * F[k];
     }
     if (rc) break;
+    if (ph->limiter && (rc = orc_limit(o, ph, K, x, y, etetf, w, w))) break;
     rc = orc_rhs(o, ph, K, x, y, etetf, t + dt, w, F, &ms, nthreads);
     if (rc) break;
     for (size_t k = 0; k < n; k++)
@@ -984,6 +1020,280 @@ int orc_ssp_advance(const orc_ops *o, const orc_phys *ph, int K, const double *x
   if (nsteps) *nsteps = steps;
   free(w);
   free(F);
+  return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* The Hesthaven-Warburton Euler limiter                                      */
+/* ------------------------------------------------------------------------ */
+
+/* Limiter's primitive variables (wxhesthavenwarburtoneulerlimiter.cc:534-543):
+ * no floors. */
+static void lim_primitive(double g, const double *qc, double *qp) {
+  qp[0] = qc[0];
+  qp[1] = qc[1] / qc[0];
+  qp[2] = qc[2] / qc[0];
+  qp[3] = qc[3] / qc[0];
+  qp[4] = (g - 1.) * (qc[4] - 0.5 * (qc[1] * qc[1] + qc[2] * qc[2] + qc[3] * qc[3]) / qc[0]);
+}
+
+/* Cell-average weights (wxnodaldggeometry2d.h:84-98): AVE[k] = 0.5 sum_m
+ * Mass[m][k] with Mass = (V V^T)^-1 (wxnodaldggeometry2d.cc:71-76). */
+int orc_limiter_weights(const orc_ops *o, double *ave) {
+  int Np = o->Np;
+  double *mass = ALLOC(double, Np * Np);
+  int rc = orc_invert(Np, o->VVT, mass);
+  for (int k = 0; k < Np; k++) {
+    ave[k] = 0.0;
+    for (int m = 0; m < Np; m++) ave[k] += 0.5 * mass[k + m * Np];
+  }
+  free(mass);
+  return rc ? 2 : 0;
+}
+
+/* WxHestavenWarburtonEulerLimiter::applyLimiter
+ * (wxhesthavenwarburtoneulerlimiter.cc:61-484), restated as written:
+ *  - the patch centres and areas use AVE[0..2] with the vertex nodes
+ *    Fmask[0], Fmask[Nfp], Fmask[2 Nfp] (:119-144);
+ *  - the face-node states of a boundary face are read from the last
+ *    neighbour read earlier in the face loop, or from the element itself
+ *    when there was none (:200-214 read qIn, which :221 re-points);
+ *  - the face-averaged primitives at the six face ends are read from the
+ *    face-ordered array at the element-node ids of the ends (:241-265);
+ *  - the third edge gradient uses (yv3 - yv2), (xv3 - xv2) (:301-305);
+ *  - every node is overwritten by the reconstruction (:394-466).
+ * The end-node ids index an array of 3 Nfp entries, so orders N > 4 read
+ * outside it in the reference; they are refused (11), as is a boundary
+ * face with the isentropic-vortex state, which reads three coordinates
+ * from a two-entry array there (:186-192, wxisentropicvortexbc.cc:23-25).
+ * qout may equal qin. */
+int orc_limit(const orc_ops *o, const orc_phys *ph, int K, const double *x,
+              const double *y, const int *etetf, const double *qin, double *qout) {
+  enum { M = 5 };
+  const int Np = o->Np, NpF = o->Nfp;
+  const int *F = o->fmask;
+  const double g = ph->lim_gamma;
+  if (ph->eqn != ORC_EULER) return 1;
+  if (o->N > 4) return 11;
+  double *ave = ALLOC(double, Np), *drop = ALLOC(double, Np * Np);
+  double *avc = ALLOC(double, (size_t)K * M), *avp = ALLOC(double, (size_t)K * M);
+  double *grad = ALLOC(double, (size_t)K * 8 * M);   /* E1x,E1y,E2x,E2y,E3x,E3y,C0x,C0y */
+  double *qo = ALLOC(double, (size_t)K * Np * M);
+  double *QM = ALLOC(double, 3 * NpF * M), *QP = ALLOC(double, 3 * NpF * M);
+  double *fc = ALLOC(double, 3 * NpF * M);
+  int rc = orc_limiter_weights(o, ave);
+  for (int i = 0; i < Np; i++)
+    for (int j = 0; j < Np; j++) drop[i * Np + j] = -ave[j] + (i == j ? 1. : 0.);
+  const int endIDs[6] = {F[0], F[NpF - 1], F[NpF], F[2 * NpF - 1], F[2 * NpF], F[3 * NpF - 1]};
+
+  for (int e = 0; e < K && !rc; e++) {
+    const int *cn = etetf + 6 * e;
+    const double *xe = x + (size_t)e * Np, *ye = y + (size_t)e * Np;
+    double geo[14], gn[14];
+    if ((rc = orc_elem_geom(o, xe, ye, geo))) break;
+    double xc[4] = {0, 0, 0, 0}, yc[4] = {0, 0, 0, 0}, A0[4] = {0, 0, 0, 0};
+    for (int kk = 0; kk < 3; kk++) {                         /* :121-126 */
+      xc[0] += ave[kk] * xe[F[kk * NpF]];
+      yc[0] += ave[kk] * ye[F[kk * NpF]];
+      A0[0] += 2. / 3. * ave[kk] * geo[4];
+    }
+    for (int f = 0; f < 3; f++) {                            /* :128-144 */
+      const double *nrm = geo + 5 + 3 * f;
+      if (cn[2 * f] < 0) {
+        double H1 = 2.0 * A0[0] / nrm[2];
+        xc[f + 1] = xc[0] + 2.0 * nrm[0] * H1;
+        yc[f + 1] = yc[0] + 2.0 * nrm[1] * H1;
+      } else {
+        const int nb = cn[2 * f];
+        const double *xn = x + (size_t)nb * Np, *yn = y + (size_t)nb * Np;
+        if ((rc = orc_elem_geom(o, xn, yn, gn))) break;
+        for (int node = 0; node < 3; node++) {
+          xc[f + 1] += ave[node] * xn[F[node * NpF]];
+          yc[f + 1] += ave[node] * yn[F[node * NpF]];
+          A0[f + 1] += 2. / 3. * ave[node] * gn[4];
+        }
+      }
+    }
+    if (rc) break;
+    for (int f = 0; f < 3; f++) A0[f + 1] += A0[0];
+
+    /* step 2: patch averages (:149-233) */
+    double cons[M][4], prim[M][4], qc[M], qp[M], qb[M];
+    const double *qIn = qin + (size_t)e * Np * M;
+    for (int c = 0; c < M; c++) {
+      qc[c] = 0.0;
+      for (int n = 0; n < Np; n++) qc[c] += ave[n] * qIn[n * M + c];
+    }
+    lim_primitive(g, qc, qp);
+    for (int c = 0; c < M; c++) {
+      cons[c][0] = qc[c];
+      prim[c][0] = qp[c];
+      avc[e * M + c] = qc[c];
+      avp[e * M + c] = qp[c];
+    }
+    for (int f = 0; f < 3; f++)
+      for (int n = 0; n < NpF; n++)
+        for (int c = 0; c < M; c++) QM[(f * NpF + n) * M + c] = qIn[F[f * NpF + n] * M + c];
+    for (int f = 0; f < 3 && !rc; f++) {
+      const double *nrm = geo + 5 + 3 * f;
+      if (cn[2 * f] < 0) {
+        const int bcnum = abs(cn[2 * f + 1]);
+        const int bi = bcnum - 1;
+        if (bi >= 0 && bi < ph->nbc && ph->bc_kind[bi] == ORC_BC_VORTEX) { rc = 11; break; }
+        double NX[2] = {nrm[0], nrm[1]}, XC[3] = {0, xc[f + 1], yc[f + 1]};
+        for (int c = 0; c < M; c++) qc[c] = cons[c][0];
+        orc_bc(ph, bcnum, XC, NX, qc, qb);
+        lim_primitive(g, qb, qp);
+        for (int c = 0; c < M; c++) {
+          cons[c][f + 1] = qb[c];
+          prim[c][f + 1] = qp[c];
+        }
+        for (int n = 0; n < NpF; n++) {
+          const int node = F[f * NpF + n];
+          double XN[3] = {0, xe[node], ye[node]};
+          for (int c = 0; c < M; c++) qc[c] = qIn[node * M + c];
+          orc_bc(ph, bcnum, XN, NX, qc, qb);
+          for (int c = 0; c < M; c++) QP[(f * NpF + n) * M + c] = qb[c];
+        }
+      } else {
+        qIn = qin + (size_t)cn[2 * f] * Np * M;
+        for (int c = 0; c < M; c++) {
+          qc[c] = 0.0;
+          for (int n = 0; n < Np; n++) qc[c] += ave[n] * qIn[n * M + c];
+        }
+        lim_primitive(g, qc, qp);
+        for (int c = 0; c < M; c++) {
+          cons[c][f + 1] = qc[c];
+          prim[c][f + 1] = qp[c];
+        }
+        const int fn = cn[2 * f + 1];
+        for (int n = 0; n < NpF; n++)
+          for (int c = 0; c < M; c++)
+            QP[(f * NpF + n) * M + c] = qIn[F[fn * NpF + NpF - 1 - n] * M + c];
+      }
+    }
+    if (rc) break;
+
+    /* step 3: face averages and their primitives at the face ends (:236-266) */
+    for (int k = 0; k < 3 * NpF * M; k++) fc[k] = 0.5 * (QP[k] + QM[k]);
+    double fp[6][M];
+    for (int n = 0; n < 6; n++) {
+      const double *a = fc + endIDs[n] * M;
+      fp[n][0] = a[0];
+      fp[n][1] = a[1] / a[0];
+      fp[n][2] = a[2] / a[0];
+      fp[n][3] = a[3] / a[0];
+      fp[n][4] = (g - 1.) * (a[4] - 0.5 * (a[1] * a[1] + a[2] * a[2] + a[3] * a[3]) / a[0]);
+    }
+
+    /* step 4: edge and cell gradients of the primitives (:269-314) */
+    const double xv1 = xe[endIDs[0]], yv1 = ye[endIDs[0]];
+    const double xv2 = xe[endIDs[2]], yv2 = ye[endIDs[2]];
+    const double xv3 = xe[endIDs[4]], yv3 = ye[endIDs[4]];
+    double *gr = grad + (size_t)e * 8 * M;
+    for (int c = 0; c < M; c++) {
+      double e1x = 0.5 * ((prim[c][1] - prim[c][0]) * (yv2 - yv1) + (fp[0][c] - fp[1][c]) * (yc[1] - yc[0])) / A0[1];
+      double e1y = -0.5 * ((prim[c][1] - prim[c][0]) * (xv2 - xv1) + (fp[0][c] - fp[1][c]) * (xc[1] - xc[0])) / A0[1];
+      double e2x = 0.5 * ((prim[c][2] - prim[c][0]) * (yv3 - yv2) + (fp[2][c] - fp[3][c]) * (yc[2] - yc[0])) / A0[2];
+      double e2y = -0.5 * ((prim[c][2] - prim[c][0]) * (xv3 - xv2) + (fp[2][c] - fp[3][c]) * (xc[2] - xc[0])) / A0[2];
+      double e3x = 0.5 * ((prim[c][3] - prim[c][0]) * (yv3 - yv2) + (fp[4][c] - fp[5][c]) * (yc[3] - yc[0])) / A0[3];
+      double e3y = -0.5 * ((prim[c][3] - prim[c][0]) * (xv3 - xv2) + (fp[4][c] - fp[5][c]) * (xc[3] - xc[0])) / A0[3];
+      gr[0 * M + c] = e1x;
+      gr[1 * M + c] = e1y;
+      gr[2 * M + c] = e2x;
+      gr[3 * M + c] = e2y;
+      gr[4 * M + c] = e3x;
+      gr[5 * M + c] = e3y;
+      gr[6 * M + c] = (A0[1] * e1x + A0[2] * e2x + A0[3] * e3x) / (A0[1] + A0[2] + A0[3]);
+      gr[7 * M + c] = (A0[1] * e1y + A0[2] * e2y + A0[3] * e3y) / (A0[1] + A0[2] + A0[3]);
+    }
+  }
+
+  /* second pass: limited gradients and reconstruction (:317-467) */
+  for (int e = 0; e < K && !rc; e++) {
+    const int *cn = etetf + 6 * e;
+    const double *xe = x + (size_t)e * Np, *ye = y + (size_t)e * Np;
+    double ldx[16], ldy[16], qdv[16 * M];
+    for (int i = 0; i < Np; i++) {
+      ldx[i] = 0.0;
+      ldy[i] = 0.0;
+    }
+    for (int i = 0; i < Np; i++)
+      for (int j = 0; j < Np; j++) {
+        ldx[i] += drop[i * Np + j] * xe[j];
+        ldy[i] += drop[i * Np + j] * ye[j];
+      }
+    for (int c = 0; c < M; c++) {
+      double gx[3], gy[3];
+      for (int f = 0; f < 3; f++) {
+        if (cn[2 * f] < 0) {
+          gx[f] = grad[((size_t)e * 8 + 2 * f) * M + c];
+          gy[f] = grad[((size_t)e * 8 + 2 * f + 1) * M + c];
+        } else {
+          gx[f] = grad[((size_t)cn[2 * f] * 8 + 6) * M + c];
+          gy[f] = grad[((size_t)cn[2 * f] * 8 + 7) * M + c];
+        }
+      }
+      double g1 = gx[0] * gx[0] + gy[0] * gy[0];
+      double g2 = gx[1] * gx[1] + gy[1] * gy[1];
+      double g3 = gx[2] * gx[2] + gy[2] * gy[2];
+      double fac = g1 * g1 + g2 * g2 + g3 * g3;
+      double epse = 1.e-14;
+      double fac3e = fac + 3.0 * epse;
+      double w1 = (g2 * g3 + epse) / fac3e;
+      double w2 = (g1 * g3 + epse) / fac3e;
+      double w3 = (g1 * g2 + epse) / fac3e;
+      double Lx = w1 * gx[0] + w2 * gx[1] + w3 * gx[2];
+      double Ly = w1 * gy[0] + w2 * gy[1] + w3 * gy[2];
+      for (int n = 0; n < Np; n++) qdv[n * M + c] = ldx[n] * Lx + ldy[n] * Ly;
+    }
+    const double *AP = avp + (size_t)e * M, *AC = avc + (size_t)e * M;
+    double *out = qo + (size_t)e * Np * M;
+    for (int n = 0; n < Np; n++) {
+      double averho = AP[0], aveu = AP[1], avev = AP[2], avew = AP[3], avep = AP[4];
+      double drho = qdv[n * M + 0], du = qdv[n * M + 1], dv = qdv[n * M + 2];
+      double dw = qdv[n * M + 3], dp = qdv[n * M + 4];
+      double Lrho = averho + drho, Lrhou, Lrhov, Lrhow, LEner, Lp;
+      if (Lrho < ph->lim_min_dens) {
+        for (int it = 0; it < 4; it++) {
+          drho *= 0.5;
+          Lrho = averho + drho;
+          if (Lrho > ph->lim_min_dens) break;
+          else if (it == 3) Lrho = AC[0];
+        }
+      }
+      Lrhou = AC[1] + averho * du + drho * aveu;
+      Lrhov = AC[2] + averho * dv + drho * avev;
+      Lrhow = AC[3] + averho * dw + drho * avew;
+      double dEner = dp / (g - 1.) + 0.5 * drho * (aveu * aveu + avev * avev + avew * avew) +
+                     averho * (aveu * du + avev * dv + avew * dw);
+      LEner = AC[4] + dEner;
+      Lp = (g - 1.) * (LEner - 0.5 * (Lrhou * Lrhou + Lrhov * Lrhov + Lrhow * Lrhow) / Lrho);
+      if (Lp < ph->lim_min_pres) {
+        for (int it = 0; it < 4; it++) {
+          dp *= 0.5;
+          dEner = dp / (g - 1.) + 0.5 * drho * (aveu * aveu + avev * avev + avew * avew) +
+                  averho * (aveu * du + avev * dv + avew * dw);
+          LEner = AC[4] + dEner;
+          Lp = (g - 1.) * (LEner - 0.5 * (Lrhou * Lrhou + Lrhov * Lrhov + Lrhow * Lrhow) / Lrho);
+          if (Lp > ph->lim_min_pres) break;
+          else if (it == 3)
+            LEner = avep / (g - 1.) + 0.5 * (Lrhou * Lrhou + Lrhov * Lrhov + Lrhow * Lrhow) / Lrho;
+        }
+      }
+      out[n * M + 0] = Lrho;
+      out[n * M + 1] = Lrhou;
+      out[n * M + 2] = Lrhov;
+      out[n * M + 3] = Lrhow;
+      out[n * M + 4] = LEner;
+    }
+  }
+  if (!rc) {
+    for (size_t i = 0; i < (size_t)K * Np * M; i++)
+      if (isnan(qo[i]) || isinf(qo[i])) { rc = 7; break; }   /* wxnodaldg2dmethod.cc:246 */
+    if (!rc) memcpy(qout, qo, sizeof(double) * (size_t)K * Np * M);
+  }
+  free(ave); free(drop); free(avc); free(avp); free(grad); free(qo); free(QM); free(QP); free(fc);
   return rc;
 }
 

@@ -69,10 +69,12 @@ int orc_elem_geom(const orc_ops *ops, const double *x, const double *y, double *
 /* Physics configuration of the RHS. */
 enum { ORC_EULER = 0, ORC_PHMAXWELL = 1 };
 /* Euler numerical fluxes follow WxEulerEqn::DGnumericalFlux
- * (wxeulereqn.cc:873-894): LF, HLL, Roe ("Wave" falls through to LF).
+ * (wxeulereqn.cc:873-894): LF, HLL, Roe, and "Wave", which falls through to
+ * LF of the face-frame traces.
  * Maxwell: LF, or the build's exact upwind flux (UPWIND). */
 enum { ORC_FLUX_LF = 0, ORC_FLUX_UPWIND = 1, ORC_FLUX_HLL = 2, ORC_FLUX_ROE = 3, ORC_FLUX_WAVE = 4 };
-enum { ORC_BC_NONE = 0, ORC_BC_VORTEX = 1, ORC_BC_TM = 2, ORC_BC_PEC = 3, ORC_BC_COPY = 4 };
+enum { ORC_BC_NONE = 0, ORC_BC_VORTEX = 1, ORC_BC_TM = 2, ORC_BC_PEC = 3, ORC_BC_COPY = 4,
+       ORC_BC_EULER_WALL = 5, ORC_BC_EULER_INFLOW = 6, ORC_BC_EULER_DIRICHLET = 7 };
 typedef struct {
   int eqn;              /* ORC_EULER (m = 5) or ORC_PHMAXWELL (m = 8)          */
   int flux;             /* ORC_FLUX_*: UPWIND for Maxwell (build ext), HLL/ROE/WAVE Euler */
@@ -81,6 +83,9 @@ typedef struct {
   int nbc;
   int bc_kind[8];
   double bc_p[8][8];
+  /* eulerLimiterHW block: gas_gamma, min_density, min_pressure */
+  double lim_gamma, lim_min_dens, lim_min_pres;
+  int limiter;          /* 1: limit each RHS input first (wxnodaldg2dmethod.cc:243-247) */
 } orc_phys;
 
 int orc_meqn(const orc_phys *ph);
@@ -92,7 +97,11 @@ int orc_flux(const orc_phys *ph, int d, const double *q, double *f);
  * wxphmaxwelleqn.cc:500-522).  qM, qP may be modified (wxeulereqn.cc:1329-1340). */
 int orc_numflux(const orc_phys *ph, const double *n, double *qM, double *qP,
                 double *nflux, double *lambda);
-/* Boundary state (wxgridbc.cc:21-26 and the three applyBC files). */
+/* Boundary state (wxgridbc.cc:21-26 and the applyBC files): isentropic
+ * vortex, Maxwell TM, PEC, and the Euler wall, inflow (p = Mach, angle in
+ * degrees, gas_gamma, pressure, density), Dirichlet (p = density, u, v, w,
+ * gas_gamma, pressure) and zero-gradient (copy) states.
+ * pinned: closed forms and reflection invariants in the tests. */
 void orc_bc(const orc_phys *ph, int bcnum, const double *xc, const double *n,
             const double *qM, double *qP);
 
@@ -115,6 +124,16 @@ void orc_set_exact_geometry(int on);
  * against itself (wxeulereqn.cc:95-200) / c0 max(chi,gamma,1) for Maxwell. */
 double orc_initial_dt(const orc_ops *ops, const orc_phys *ph, int K,
                       const double *q, double cfl, double dtscale);
+
+/* Cell-average weights of the limiter, ave[Np] (wxnodaldggeometry2d.h:84-98). */
+int orc_limiter_weights(const orc_ops *ops, double *ave);
+/* Hesthaven-Warburton Euler limiter, applyLimiter as written
+ * (wxhesthavenwarburtoneulerlimiter.cc:61-484); Euler, N <= 4, no vortex
+ * boundary (11 otherwise); 7 on a NaN/Inf result.  qout may equal qin.
+ * pinned: preserves cell averages and linear states away from the patch
+ * quirks, reproduces by hand-computed patches, see tests. */
+int orc_limit(const orc_ops *ops, const orc_phys *ph, int K, const double *x,
+              const double *y, const int *etetf, const double *qin, double *qout);
 
 /* LSERK4 (Carpenter & Kennedy 1994 five-stage, as tabulated in Hesthaven &
  * Warburton 2008): no reference counterpart. */

@@ -18,6 +18,7 @@ _lib = None
 ORC_EULER, ORC_PHMAXWELL = 0, 1
 ORC_FLUX_LF, ORC_FLUX_UPWIND, ORC_FLUX_HLL, ORC_FLUX_ROE, ORC_FLUX_WAVE = 0, 1, 2, 3, 4
 ORC_BC_NONE, ORC_BC_VORTEX, ORC_BC_TM, ORC_BC_PEC, ORC_BC_COPY = 0, 1, 2, 3, 4
+ORC_BC_EULER_WALL, ORC_BC_EULER_INFLOW, ORC_BC_EULER_DIRICHLET = 5, 6, 7
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -40,7 +41,9 @@ class _Phys(ctypes.Structure):
                 ("min_pres", ctypes.c_double), ("c0", ctypes.c_double),
                 ("chi", ctypes.c_double), ("gph", ctypes.c_double),
                 ("nbc", ctypes.c_int), ("bc_kind", ctypes.c_int * 8),
-                ("bc_p", (ctypes.c_double * 8) * 8)]
+                ("bc_p", (ctypes.c_double * 8) * 8),
+                ("lim_gamma", ctypes.c_double), ("lim_min_dens", ctypes.c_double),
+                ("lim_min_pres", ctypes.c_double), ("limiter", ctypes.c_int)]
 
 
 def lib():
@@ -73,6 +76,9 @@ def lib():
                                  ctypes.c_int, _dp, ctypes.c_int]
         L.orc_adjoint_rhs.argtypes = [ctypes.POINTER(_Ops), ctypes.POINTER(_Phys), ctypes.c_int,
                                       _dp, _dp, _ip, ctypes.c_double, _dp, _dp, _dp]
+        L.orc_limit.argtypes = [ctypes.POINTER(_Ops), ctypes.POINTER(_Phys), ctypes.c_int,
+                                _dp, _dp, _ip, _dp, _dp]
+        L.orc_limiter_weights.argtypes = [ctypes.POINTER(_Ops), _dp]
         L.orc_ssp_advance.argtypes = [ctypes.POINTER(_Ops), ctypes.POINTER(_Phys), ctypes.c_int,
                                       _dp, _dp, _ip, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -148,9 +154,13 @@ class Ops:
 
 
 def phys(eqn, flux=ORC_FLUX_LF, gamma=1.4, min_dens=0.0, min_pres=0.0, c0=1.0, chi=0.0,
-         gph=0.0, bcs=()):
-    """bcs: sequence of (kind, params) indexed by tag-1."""
+         gph=0.0, bcs=(), limiter=None):
+    """bcs: sequence of (kind, params) indexed by tag-1.  limiter: None, or
+    (gas_gamma, min_density, min_pressure) of an eulerLimiterHW block."""
     p = _Phys()
+    if limiter is not None:
+        p.limiter = 1
+        p.lim_gamma, p.lim_min_dens, p.lim_min_pres = limiter
     p.eqn, p.flux = eqn, flux
     p.gamma, p.min_dens, p.min_pres = gamma, min_dens, min_pres
     p.c0, p.chi, p.gph = c0, chi, gph
@@ -301,6 +311,25 @@ def ssp_advance(mesh, ph, t0, t_end, dt0, cfl, q, stages=5, max_steps=100000, nt
     if rc:
         raise ValueError("orc_ssp_advance: %d" % rc)
     return q, dts[:n.value].copy()
+
+
+def limiter_weights(ops):
+    ave = np.zeros(ops.Np)
+    rc = lib().orc_limiter_weights(ctypes.byref(ops._o), _d(ave))
+    if rc:
+        raise ValueError("orc_limiter_weights: %d" % rc)
+    return ave
+
+
+def limit(mesh, ph, q):
+    """Apollo's Hesthaven-Warburton Euler limiter applied to q (a copy)."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    out = np.zeros_like(q)
+    rc = lib().orc_limit(ctypes.byref(mesh.ops._o), ctypes.byref(ph), mesh.K, _d(mesh.x),
+                         _d(mesh.y), _i(mesh.etetf), _d(q), _d(out))
+    if rc:
+        raise ValueError("orc_limit: %d" % rc)
+    return out
 
 
 def adjoint_rhs(mesh, ph, t, q, lam):

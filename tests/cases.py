@@ -48,7 +48,7 @@ def oracle_phys(params):
             E.FLUX_ROE: O.ORC_FLUX_ROE, E.FLUX_WAVE: O.ORC_FLUX_WAVE}[params.flux]
     return O.phys(eqn, flux=flux, gamma=params.gas_gamma, min_dens=params.min_density,
                   min_pres=params.min_pressure, c0=params.c0, chi=params.chi,
-                  gph=params.gamma_ph, bcs=params.bcs)
+                  gph=params.gamma_ph, bcs=params.bcs, limiter=params.limiter)
 
 
 class Case:
@@ -129,6 +129,48 @@ def euler_case(n=16, N=3, m=4, periodic=True, precision=E.PREC_FP64, L=10.0, see
         mesh = E.structured_mesh(n, -L / 2, L / 2, -L / 2, L / 2, 0.1, seed, False)
         bcs = [vortex_bc()]
     return Case(mesh, E.Params(N, eqn, flux=flux, precision=precision, bcs=bcs), "euler")
+
+
+# Apollo's forward-facing-step deck (examples/unstructuredDG/euler/
+# forwardFacingStep/forwardFacingStep.pin): Mach 3 inflow of rho = gamma,
+# p = 5 over the step mesh grid.msh, boundary list [inflow, bottomWall,
+# topWall, noGrad], HLL flux, eulerLimiterHW with min_pressure =
+# min_density = 1e-2, cfl = 1/(2(N+1)-1).
+FFS_MACH, FFS_P, FFS_GAMMA = 3.0, 5.0, 1.4
+
+
+def ffs_case(N=1, m=5, flux=E.FLUX_HLL, limiter=True, precision=E.PREC_FP64):
+    from apollo_amd.gmsh import read_gmsh
+    mesh = read_gmsh(os.path.join(GOLDEN, "meshes", "ffs_grid.msh"))
+    inflow = (E.BC_EULER_INFLOW, [FFS_MACH, 0.0, FFS_GAMMA, FFS_P, FFS_GAMMA])
+    bcs = [inflow, (E.BC_EULER_WALL, []), (E.BC_EULER_WALL, []), (E.BC_COPY, [])]
+    eqn = E.EQN_EULER5 if m == 5 else E.EQN_EULER4
+    lim = (FFS_GAMMA, 1e-2, 1e-2) if limiter else None
+    return Case(mesh, E.Params(N, eqn, flux=flux, precision=precision, gas_gamma=FFS_GAMMA,
+                               bcs=bcs, limiter=lim), "ffs")
+
+
+def ffs_state(case, kind="uniform", seed=0):
+    """The deck's uniform Mach-3 state, or that state with a smooth
+    perturbation of amplitude 10% (still positive)."""
+    x, y = case.omesh.x, case.omesh.y
+    g, p0 = FFS_GAMMA, FFS_P
+    rho = np.full_like(x, g)
+    u = np.full_like(x, FFS_MACH * np.sqrt(g * p0 / g))
+    v = np.zeros_like(x)
+    p = np.full_like(x, p0)
+    if kind != "uniform":
+        rng = np.random.default_rng(seed)
+        for arr, amp in ((rho, 0.1), (u, 0.1), (v, 0.1), (p, 0.1)):
+            k = rng.standard_normal(4)
+            arr *= 1.0 + amp * np.sin(k[0] * 3 * x + k[1] * 3 * y + k[2]) * np.cos(k[3] * 2 * y)
+            if arr is v:
+                arr += amp * np.sin(2 * x + 3 * y + k[2])
+    q = np.zeros(x.shape + (case.params.m,))
+    E_ = p / (g - 1) + 0.5 * rho * (u * u + v * v)
+    q[..., 0], q[..., 1], q[..., 2] = rho, rho * u, rho * v
+    q[..., -1] = E_
+    return np.ascontiguousarray(q)
 
 
 def maxwell_case(n=12, N=5, m=3, bc=E.BC_MAXWELL_TM, flux=E.FLUX_LF, precision=E.PREC_FP64,

@@ -7,7 +7,7 @@ Reads reference data files as text (nothing is compiled, imported or run):
   nodes p_r/p_s, Vandermonde p_Vand and its inverse p_IVand (N=1 only), nodal
   differentiation p_Dr/p_Ds, the strong-form lift p_LIFT and NUDG's face
   masks p_Fmask.  Stored as ``nudg_N<N>.npz`` (plain arrays, no pickle).
-* four Gmsh v2.2 meshes from ``examples/unstructuredDG`` copied verbatim into
+* five Gmsh v2.2 meshes from ``examples/unstructuredDG`` copied verbatim into
   ``meshes/`` (data the reference ships with its example decks).
 
 Usage: python tests/golden/make_fixtures.py [/root/reference]
@@ -28,6 +28,7 @@ MESHES = {
     "small_square_4bc.msh": "examples/unstructuredDG/multifluid/collisionalRadiative/small_square_4bc.msh",
     "vortex_grid.msh": "examples/unstructuredDG/euler/isentropicVortex/grid.msh",
     "tm_square.msh": "examples/unstructuredDG/maxwell/transverseMagnetic/square.msh",
+    "ffs_grid.msh": "examples/unstructuredDG/euler/forwardFacingStep/grid.msh",
 }
 
 

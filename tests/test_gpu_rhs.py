@@ -9,8 +9,8 @@ import pytest
 from apollo_amd import engine as E
 from oracle import oracle as O
 
-from cases import (Case, euler_case, fp64_parity, maxwell_case, rel_l2, vortex_bc,
-                   walled_unit_square)
+from cases import (Case, euler_case, ffs_case, ffs_state, fp64_parity, maxwell_case, rel_l2,
+                   vortex_bc, walled_unit_square)
 
 pytestmark = pytest.mark.gpu
 
@@ -103,6 +103,22 @@ def test_euler_riemann_fluxes_reduced_precision(flux):
     e3 = rel_l2(_rhs(case, q, precision=E.PREC_BF16X3), ref)
     print("flux %d: FP32 %.3e BF16x3 %.3e" % (flux, e32, e3))
     assert e32 <= 2e-5 and e3 <= 1e-3
+
+
+@pytest.mark.parametrize("flux", [E.FLUX_LF, E.FLUX_HLL])
+@pytest.mark.parametrize("N", [1, 3, 6])
+def test_euler_wall_inflow_outflow_rhs_fp64(flux, N):
+    """Apollo's forward-facing-step mesh with its inflow, wall and
+    zero-gradient boundaries, and a Dirichlet-walled square."""
+    for m in (4, 5):
+        case = ffs_case(N=N, m=m, flux=flux, limiter=False)
+        q = ffs_state(case, "smooth", seed=N)
+        fp64_parity(case, _rhs(case, q), q)
+    sq = euler_case(n=6, N=N, m=5, periodic=False, flux=flux)
+    case = Case(sq.mesh, E.Params(N, E.EQN_EULER5, flux=flux,
+                                  bcs=[(E.BC_EULER_DIRICHLET, [1.0, 0.5, 0.1, 0.05, 1.4, 1.0])]))
+    q = sq.state()
+    fp64_parity(case, _rhs(case, q), q)
 
 
 @pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6, 7, 8])

@@ -43,6 +43,13 @@ def test_create_validates_before_touching_the_device():
         (E.Params(3, 7), 1),                                                 # eqn
         (E.Params(3, E.EQN_EULER4, flux=E.FLUX_UPWIND), 1),                  # upwind: TM only
         (E.Params(3, E.EQN_MAXWELL_TM3, chi=1.0), 1),                        # TM needs chi=0
+        # the limiter: Euler only, N <= 4, no isentropic-vortex boundary
+        (E.Params(3, E.EQN_MAXWELL_TM3, limiter=(1.4, 0.0, 0.0)), 1),
+        (E.Params(5, E.EQN_EULER5, limiter=(1.4, 0.0, 0.0)), 11),
+        (E.Params(3, E.EQN_EULER5, limiter=(1.4, 0.0, 0.0),
+                  bcs=[(E.BC_ISENTROPIC_VORTEX, [1, 0, 0, 0, 1.4, 5])]), 11),
+        # EULER4 carries no z momentum
+        (E.Params(3, E.EQN_EULER4, bcs=[(E.BC_EULER_DIRICHLET, [1, 0, 0, 0.5, 1.4, 1])]), 1),
     ]
     for params, code in bad:
         with pytest.raises(E.ApolloError) as ei:
@@ -84,7 +91,7 @@ def test_host_connectivity_matches_oracle(periodic):
 
 
 @pytest.mark.parametrize("name", ["advection_test.msh", "small_square_4bc.msh",
-                                  "vortex_grid.msh", "tm_square.msh"])
+                                  "vortex_grid.msh", "tm_square.msh", "ffs_grid.msh"])
 def test_host_connectivity_gmsh_fixtures(name):
     from apollo_amd.gmsh import read_gmsh
     mesh = read_gmsh(os.path.join(GOLDEN, "meshes", name))
@@ -98,7 +105,7 @@ def test_gmsh_reader():
     assert mesh.K == 16 and mesh.Nv == 13
     assert sorted(set(mesh.face_tag.ravel()) - {0}) == [1, 2, 3, 4]
     # every triangle of Apollo's example meshes is counter-clockwise
-    for name in ("advection_test.msh", "vortex_grid.msh", "tm_square.msh"):
+    for name in ("advection_test.msh", "vortex_grid.msh", "tm_square.msh", "ffs_grid.msh"):
         m = read_gmsh(os.path.join(GOLDEN, "meshes", name))
         x, y = m.VX[m.EToV], m.VY[m.EToV]
         area = (x[:, 1] - x[:, 0]) * (y[:, 2] - y[:, 0]) - (x[:, 2] - x[:, 0]) * (y[:, 1] - y[:, 0])
