@@ -600,6 +600,13 @@ int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out)
   if ((rc = dev_alloc(h, (void **)&h->d_speed, sizeof(double) * ((h->K + 15) / 16)))) return bail(rc);
   if ((rc = dev_alloc(h, (void **)&h->d_t, sizeof(double) * 2))) return bail(rc);
   if (params->limiter) {
+    // the limiter's face ends are the vertex nodes shared by consecutive
+    // faces: Fmask[f][Nfp-1] == Fmask[f+1][0]
+    const int *F = o.fmask.data();
+    if (F[Nfp - 1] != F[Nfp] || F[2 * Nfp - 1] != F[2 * Nfp] || F[3 * Nfp - 1] != F[0]) {
+      h->last_error = "limiter: face node lists do not chain at the vertices";
+      return bail(APL_ERR_UNSUPPORTED);
+    }
     // ave[Np] = 0.5 sum_m Mass[m][k], Mass = (V V^T)^-1 (wxnodaldggeometry2d.h:84-98),
     // then r, s and drop[i][j] = delta_ij - ave[j]
     std::vector<double> mass((size_t)Np * Np), tab((size_t)3 * Np + (size_t)Np * Np);
@@ -617,7 +624,7 @@ int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out)
     for (int i = 0; i < Np; ++i)
       for (int j = 0; j < Np; ++j) tab[3 * Np + (size_t)i * Np + j] = -tab[j] + (i == j ? 1. : 0.);
     if ((rc = dev_upload<double>(h, &h->d_ltab, tab.data(), tab.size()))) return bail(rc);
-    if ((rc = dev_alloc(h, (void **)&h->d_lavg, sizeof(double) * 5 * (size_t)h->K))) return bail(rc);
+    if ((rc = dev_alloc(h, (void **)&h->d_lavg, sizeof(double) * 8 * (size_t)h->K))) return bail(rc);
     if ((rc = dev_alloc(h, (void **)&h->d_lgrad, sizeof(double) * 40 * (size_t)h->K))) return bail(rc);
   }
   *out = h;

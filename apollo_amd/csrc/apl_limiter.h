@@ -15,8 +15,11 @@
 //                           and the reconstruction with its density and
 //                           pressure floors (:317-467).
 //
-// One thread per element: each pass is a gather over a few hundred bytes
-// per element with no reuse across elements, bound by HBM traffic.  The
+// Each pass is bound by HBM traffic.  The average and reconstruction
+// passes move the state of 32 consecutive elements per wave between HBM and
+// LDS with coalesced 16-byte accesses, one lane per element on the LDS copy;
+// the gradient pass, a gather of a few hundred bytes per element, runs one
+// thread per element.  The
 // reference's reading order and quirks are kept (see the oracle's
 // orc_limit): AVE[0..2] with the vertex nodes for the patch centres, the
 // boundary-face node states read from the last neighbour visited, the face
@@ -33,8 +36,11 @@ struct LimArgs {
   int K;
   const double *qin;       // [K][Np][M]
   double *qout;            // [K][Np][M], may equal qin
-  double *avg;             // [K][5]   conserved cell averages
-  double *grad;            // [K][8][5] E1x,E1y,E2x,E2y,E3x,E3y,C0x,C0y
+  double *avg;             // [K][8]   conserved cell averages, then the
+                           //          patch centre xc, yc and area share A0
+  double *grad;            // [K][8][5] C0x,C0y,E1x,E1y,E2x,E2y,E3x,E3y; the
+                           //          edge rows only for elements with a
+                           //          boundary face (the only readers)
   const double *geom;      // [K][G_WORDS]
   const double *vert;      // [K][6]
   const int *fcode;        // [K][3]
@@ -54,13 +60,47 @@ __device__ __forceinline__ void lim_prim(double g, const double *c, double *p) {
   p[4] = (g - 1.) * (c[4] - 0.5 * (c[1] * c[1] + c[2] * c[2] + c[3] * c[3]) / c[0]);
 }
 
+// The state of LE consecutive elements moves between HBM and LDS with
+// coalesced 16-byte accesses by all 64 lanes of a wave; lane e < LE then
+// works on element e from LDS.
+constexpr int LE = 32;
+
+template <int ES>
+__device__ __forceinline__ void lim_stage_in(const double *src, double *lds, int cnt, int lane) {
+  const int n = cnt * ES;
+  if (cnt == LE) {
+    constexpr int NV2 = LE * ES / 2;
+#pragma unroll 4
+    for (int i = lane; i < NV2; i += 64) ((dbl2 *)lds)[i] = ((const dbl2 *)src)[i];
+  } else {
+    for (int i = lane; i < n; i += 64) lds[i] = src[i];
+  }
+}
+
+template <int ES>
+__device__ __forceinline__ void lim_stage_out(const double *lds, double *dst, int cnt, int lane) {
+  const int n = cnt * ES;
+  if (cnt == LE) {
+    constexpr int NV2 = LE * ES / 2;
+#pragma unroll 4
+    for (int i = lane; i < NV2; i += 64) ((dbl2 *)dst)[i] = ((const dbl2 *)lds)[i];
+  } else {
+    for (int i = lane; i < n; i += 64) dst[i] = lds[i];
+  }
+}
+
 template <int N, int EQN>
-__global__ void __launch_bounds__(256) lim_average_kernel(LimArgs a) {
-  constexpr int Np = Order<N>::Np, M = Vars<EQN>::M;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= a.K) return;
+__global__ void __launch_bounds__(64) lim_average_kernel(LimArgs a) {
+  constexpr int Np = Order<N>::Np, M = Vars<EQN>::M, ES = Np * M;
+  __shared__ __align__(16) double st[LE * ES];
+  const int lane = threadIdx.x;
+  const long e0 = (long)blockIdx.x * LE;
+  const int cnt = (int)((a.K - e0) < LE ? (a.K - e0) : LE);
+  lim_stage_in<ES>(a.qin + e0 * ES, st, cnt, lane);
+  __syncthreads();
+  if (lane >= cnt) return;
   const double *ave = a.tab;
-  const double *q = a.qin + (long)e * Np * M;
+  const double *q = st + lane * ES;
   double c5[5];
 #pragma unroll
   for (int c = 0; c < 5; ++c) c5[c] = 0.0;
@@ -71,8 +111,27 @@ __global__ void __launch_bounds__(256) lim_average_kernel(LimArgs a) {
 #pragma unroll
     for (int c = 0; c < 5; ++c) c5[c] += ave[n] * q5[c];
   }
+  // the element's patch centre and area share (:121-126), which its
+  // neighbours' patches reuse (:136-144): same arithmetic either way
+  const long e = e0 + lane;
+  const double *vk = a.vert + e * 6;
+  // J from the vertices as the geometry computes it (wxnodaldggeometry2d.cc:412-426)
+  const double xr = (vk[2] - vk[0]) / 2, xs = (vk[4] - vk[0]) / 2;
+  const double yr = (vk[3] - vk[1]) / 2, ys = (vk[5] - vk[1]) / 2;
+  const double J = -xs * yr + xr * ys;
+  double xc = 0.0, yc = 0.0, A0 = 0.0;
 #pragma unroll
-  for (int c = 0; c < 5; ++c) a.avg[(long)e * 5 + c] = c5[c];
+  for (int kk = 0; kk < 3; ++kk) {
+    xc += ave[kk] * vk[2 * kk];
+    yc += ave[kk] * vk[2 * kk + 1];
+    A0 += 2. / 3. * ave[kk] * J;
+  }
+  double *o = a.avg + e * 8;
+#pragma unroll
+  for (int c = 0; c < 5; ++c) o[c] = c5[c];
+  o[5] = xc;
+  o[6] = yc;
+  o[7] = A0;
 }
 
 template <int N, int EQN>
@@ -81,57 +140,26 @@ __global__ void __launch_bounds__(256) lim_gradient_kernel(LimArgs a) {
   static_assert(N <= 4, "the limiter's face-end reads stay inside 3 Nfp entries for N <= 4");
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= a.K) return;
-  const double *ave = a.tab;
   const int *F = a.fmask;
   const double *gk = a.geom + (long)e * G_WORDS;
   const double *vk = a.vert + (long)e * 6;
-  const double J = gk[G_J];
   int code[3];
 #pragma unroll
   for (int f = 0; f < 3; ++f) code[f] = a.fcode[(long)e * 3 + f];
 
-  // patch centres and areas (:121-147)
-  double xc[4], yc[4], A0[4];
-  xc[0] = 0.0;
-  yc[0] = 0.0;
-  A0[0] = 0.0;
+  // patch centres and areas (:121-147) and patch averages (:149-233);
+  // src[f] is the element whose nodes the reference reads for face f's
+  // boundary states: the last neighbour visited before it, or the element
+  // itself
+  double xc[4], yc[4], A0[4], pr[4][5], own[5];
+  {
+    const double *o = a.avg + (long)e * 8;
 #pragma unroll
-  for (int kk = 0; kk < 3; ++kk) {
-    xc[0] += ave[kk] * vk[2 * kk];
-    yc[0] += ave[kk] * vk[2 * kk + 1];
-    A0[0] += 2. / 3. * ave[kk] * J;
+    for (int c = 0; c < 5; ++c) own[c] = o[c];
+    xc[0] = o[5];
+    yc[0] = o[6];
+    A0[0] = o[7];
   }
-#pragma unroll
-  for (int f = 0; f < 3; ++f) {
-    xc[f + 1] = 0.0;
-    yc[f + 1] = 0.0;
-    A0[f + 1] = 0.0;
-    if (code[f] < 0) {
-      const double H1 = 2.0 * A0[0] / gk[G_SJ + f];
-      xc[f + 1] = xc[0] + 2.0 * gk[G_NX + f] * H1;
-      yc[f + 1] = yc[0] + 2.0 * gk[G_NY + f] * H1;
-    } else {
-      const long nb = code[f] >> 2;
-      const double *vn = a.vert + nb * 6;
-      const double Jn = a.geom[nb * G_WORDS + G_J];
-#pragma unroll
-      for (int node = 0; node < 3; ++node) {
-        xc[f + 1] += ave[node] * vn[2 * node];
-        yc[f + 1] += ave[node] * vn[2 * node + 1];
-        A0[f + 1] += 2. / 3. * ave[node] * Jn;
-      }
-    }
-  }
-#pragma unroll
-  for (int f = 0; f < 3; ++f) A0[f + 1] += A0[0];
-
-  // patch averages (:149-233); src[f] is the element whose nodes the
-  // reference reads for face f's boundary states: the last neighbour
-  // visited before it, or the element itself
-  double pr[4][5];
-  double own[5];
-#pragma unroll
-  for (int c = 0; c < 5; ++c) own[c] = a.avg[(long)e * 5 + c];
   lim_prim(a.gamma, own, pr[0]);
   long src[3];
   long cur = e;
@@ -139,26 +167,38 @@ __global__ void __launch_bounds__(256) lim_gradient_kernel(LimArgs a) {
   for (int f = 0; f < 3; ++f) {
     double cf[5];
     if (code[f] < 0) {
+      const double H1 = 2.0 * A0[0] / gk[G_SJ + f];
+      xc[f + 1] = xc[0] + 2.0 * gk[G_NX + f] * H1;
+      yc[f + 1] = yc[0] + 2.0 * gk[G_NY + f] * H1;
+      A0[f + 1] = 0.0;
       const int bi = -code[f] - 1;
       const int kind = (bi >= 0 && bi < a.bc.n) ? a.bc.kind[bi] : APL_BC_COPY;
       const double *bp = a.bc.p[(bi >= 0 && bi < a.bc.n) ? bi : 0];
       euler_bc5(kind, bp, 0.0, xc[f + 1], yc[f + 1], gk[G_NX + f], gk[G_NY + f], own, cf);
     } else {
       cur = code[f] >> 2;
+      const double *o = a.avg + cur * 8;
 #pragma unroll
-      for (int c = 0; c < 5; ++c) cf[c] = a.avg[cur * 5 + c];
+      for (int c = 0; c < 5; ++c) cf[c] = o[c];
+      xc[f + 1] = o[5];
+      yc[f + 1] = o[6];
+      A0[f + 1] = o[7];
     }
     src[f] = cur;
     lim_prim(a.gamma, cf, pr[f + 1]);
   }
+#pragma unroll
+  for (int f = 0; f < 3; ++f) A0[f + 1] += A0[0];
 
   // face averages at the six face ends, read at face-node position
-  // p = element-node id of the end (:241-265)
+  // p = element-node id of the end (:241-265).  The ends are the vertex
+  // nodes (v1, v2, v2, v3, v3, v1): each face's node list starts where the
+  // previous one ends (checked on the host), so three evaluations serve.
   double fp[6][5];
-  const int ends[6] = {F[0], F[Nfp - 1], F[Nfp], F[2 * Nfp - 1], F[2 * Nfp], F[3 * Nfp - 1]};
+  const int ends[3] = {F[0], F[Nfp - 1], F[2 * Nfp - 1]};
   const double *qe = a.qin + (long)e * Np * M;
 #pragma unroll
-  for (int i = 0; i < 6; ++i) {
+  for (int i = 0; i < 3; ++i) {
     const int p = ends[i];
     const int f = p / Nfp, j = p - f * Nfp;
     double qm[5], qp[5];
@@ -180,12 +220,19 @@ __global__ void __launch_bounds__(256) lim_gradient_kernel(LimArgs a) {
     double fc[5];
 #pragma unroll
     for (int c = 0; c < 5; ++c) fc[c] = 0.5 * (qp[c] + qm[c]);
-    lim_prim(a.gamma, fc, fp[i]);
+    lim_prim(a.gamma, fc, fp[i == 0 ? 0 : 2 * i - 1]);
+  }
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    fp[2][c] = fp[1][c];
+    fp[4][c] = fp[3][c];
+    fp[5][c] = fp[0][c];
   }
 
   // edge and cell gradients of the primitives (:269-314)
   const double xv1 = vk[0], yv1 = vk[1], xv2 = vk[2], yv2 = vk[3], xv3 = vk[4], yv3 = vk[5];
   double *gr = a.grad + (long)e * 40;
+  const bool edge_rows = code[0] < 0 || code[1] < 0 || code[2] < 0;
 #pragma unroll
   for (int c = 0; c < 5; ++c) {
     const double e1x = 0.5 * ((pr[1][c] - pr[0][c]) * (yv2 - yv1) + (fp[0][c] - fp[1][c]) * (yc[1] - yc[0])) / A0[1];
@@ -195,22 +242,23 @@ __global__ void __launch_bounds__(256) lim_gradient_kernel(LimArgs a) {
     const double e3x = 0.5 * ((pr[3][c] - pr[0][c]) * (yv3 - yv2) + (fp[4][c] - fp[5][c]) * (yc[3] - yc[0])) / A0[3];
     const double e3y = -0.5 * ((pr[3][c] - pr[0][c]) * (xv3 - xv2) + (fp[4][c] - fp[5][c]) * (xc[3] - xc[0])) / A0[3];
     const double sa = A0[1] + A0[2] + A0[3];
-    gr[0 * 5 + c] = e1x;
-    gr[1 * 5 + c] = e1y;
-    gr[2 * 5 + c] = e2x;
-    gr[3 * 5 + c] = e2y;
-    gr[4 * 5 + c] = e3x;
-    gr[5 * 5 + c] = e3y;
-    gr[6 * 5 + c] = (A0[1] * e1x + A0[2] * e2x + A0[3] * e3x) / sa;
-    gr[7 * 5 + c] = (A0[1] * e1y + A0[2] * e2y + A0[3] * e3y) / sa;
+    gr[0 * 5 + c] = (A0[1] * e1x + A0[2] * e2x + A0[3] * e3x) / sa;
+    gr[1 * 5 + c] = (A0[1] * e1y + A0[2] * e2y + A0[3] * e3y) / sa;
+    if (edge_rows) {
+      gr[2 * 5 + c] = e1x;
+      gr[3 * 5 + c] = e1y;
+      gr[4 * 5 + c] = e2x;
+      gr[5 * 5 + c] = e2y;
+      gr[6 * 5 + c] = e3x;
+      gr[7 * 5 + c] = e3y;
+    }
   }
 }
 
+// One element of the reconstruction pass; writes its Np x M result to out.
 template <int N, int EQN>
-__global__ void __launch_bounds__(256) lim_reconstruct_kernel(LimArgs a) {
+__device__ __forceinline__ void lim_reconstruct_element(const LimArgs &a, long e, double *out) {
   constexpr int Np = Order<N>::Np, M = Vars<EQN>::M;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= a.K) return;
   const double *rn = a.tab + Np, *sn = a.tab + 2 * Np, *drop = a.tab + 3 * Np;
   const double *vk = a.vert + (long)e * 6;
   int code[3];
@@ -224,7 +272,7 @@ __global__ void __launch_bounds__(256) lim_reconstruct_kernel(LimArgs a) {
     double gx[3], gy[3];
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
-      const long row = code[f] < 0 ? (long)e * 8 + 2 * f : (long)(code[f] >> 2) * 8 + 6;
+      const long row = code[f] < 0 ? (long)e * 8 + 2 + 2 * f : (long)(code[f] >> 2) * 8;
       gx[f] = a.grad[row * 5 + c];
       gy[f] = a.grad[(row + 1) * 5 + c];
     }
@@ -252,12 +300,11 @@ __global__ void __launch_bounds__(256) lim_reconstruct_kernel(LimArgs a) {
 
   double AC[5], AP[5];
 #pragma unroll
-  for (int c = 0; c < 5; ++c) AC[c] = a.avg[(long)e * 5 + c];
+  for (int c = 0; c < 5; ++c) AC[c] = a.avg[(long)e * 8 + c];
   lim_prim(a.gamma, AC, AP);
   const double g = a.gamma;
   const double averho = AP[0], aveu = AP[1], avev = AP[2], avew = AP[3], avep = AP[4];
   unsigned ef = 0;
-  double *out = a.qout + (long)e * Np * M;
 #pragma unroll 1
   for (int n = 0; n < Np; ++n) {
     double ldx = 0.0, ldy = 0.0;
@@ -308,6 +355,18 @@ __global__ void __launch_bounds__(256) lim_reconstruct_kernel(LimArgs a) {
     store_euler5<EQN>(o5, out + n * M);
   }
   if (ef) atomicOr(a.err, ef);
+}
+
+template <int N, int EQN>
+__global__ void __launch_bounds__(64) lim_reconstruct_kernel(LimArgs a) {
+  constexpr int Np = Order<N>::Np, M = Vars<EQN>::M, ES = Np * M;
+  __shared__ __align__(16) double st[LE * ES];
+  const int lane = threadIdx.x;
+  const long e0 = (long)blockIdx.x * LE;
+  const int cnt = (int)((a.K - e0) < LE ? (a.K - e0) : LE);
+  if (lane < cnt) lim_reconstruct_element<N, EQN>(a, e0 + lane, st + lane * ES);
+  __syncthreads();
+  lim_stage_out<ES>(st, a.qout + e0 * ES, cnt, lane);
 }
 
 // Launch entry of one order (defined in apl_kernels_N<N>.hip for N <= 4).

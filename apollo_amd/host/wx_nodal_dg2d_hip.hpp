@@ -66,6 +66,11 @@ struct NodalDG2dConfig {
   double c0 = 1.0, chi = 0.0, gamma = 0.0;
   int precision = APL_PREC_FP64;
   int algorithm = APL_ALGO_AUTO;
+  // `Limiter = [<eulerLimiterHW block>]` (wxnodaldg2dmethod.cc:126-134) and
+  // that block's gas_gamma, min_density, min_pressure
+  // (wxhesthavenwarburtoneulerlimiter.cc:21-23)
+  bool limiter = false;
+  double limiterGasGamma = 1.4, limiterMinDensity = 0.0, limiterMinPressure = 0.0;
   std::vector<apl_bc> boundaryConditions;   // entry T-1 serves "Face Sets" tag T
 };
 
@@ -110,6 +115,10 @@ class WxNodalDG2dMethodHip {
     p.c0 = cfg.c0;
     p.chi = cfg.chi;
     p.gamma_ph = cfg.gamma;
+    p.limiter = cfg.limiter ? 1 : 0;
+    p.lim_gas_gamma = cfg.limiterGasGamma;
+    p.lim_min_density = cfg.limiterMinDensity;
+    p.lim_min_pressure = cfg.limiterMinPressure;
     p.n_bc = (int)cfg.boundaryConditions.size();
     if (p.n_bc > APL_MAX_BC) throw WxExcept(APL_ERR_ARG, "too many boundary conditions");
     for (int i = 0; i < p.n_bc; ++i) p.bc[i] = cfg.boundaryConditions[i];
@@ -133,13 +142,21 @@ class WxNodalDG2dMethodHip {
     return dt;
   }
 
-  // The RHS of one stage (wxnodaldg2dmethod.cc:235-542): `out` is
-  // overwritten; the status carries (2/3) cfl dtscale rmin / maxSpeed.
-  WxStepperStatus<double> step(double t, double /*dt*/, const double *in, double *out,
+  // The RHS of one stage (wxnodaldg2dmethod.cc:235-542): with a limiter,
+  // `in` is limited in place first (:243-247), as Apollo does to the vector
+  // it is handed; then `out` is overwritten with the RHS.  The status
+  // carries (2/3) cfl dtscale rmin / maxSpeed.
+  WxStepperStatus<double> step(double t, double /*dt*/, double *in, double *out,
                                int where = APL_HOST) {
+    if (cfg_.limiter) applyLimiter(in, in, where);
     double sdt = 0.0;
     check(apl_rhs(h_, t, in, out, cfg_.cfl, &sdt, where), "step");
     return WxStepperStatus<double>(true, sdt);
+  }
+
+  // WxNodalDG2dMethod::applyLimiter (wxnodaldg2dmethod.cc:575-582).
+  void applyLimiter(const double *qin, double *qlimited, int where = APL_HOST) {
+    check(apl_limit(h_, qin, qlimited, where), "applyLimiter");
   }
 
   // nsteps fixed-dt LSERK4 steps in place (one hipGraph per step).
@@ -180,7 +197,7 @@ class WxNodalDG2dMethodHip {
   // per-step subsolver's RHS, keep the smallest suggested dt.
   static WxStepperStatus<double> computeRHSforTS(
       const std::vector<WxNodalDG2dMethodHip *> &subsolvers, double t, double dt,
-      const double *in, double *out, size_t n) {
+      double *in, double *out, size_t n) {
     std::vector<double> x(n);
     for (size_t i = 0; i < n; ++i) out[i] = 0.0;
     double dt_temp = dt;

@@ -81,6 +81,16 @@ def workload(name):
                 "seed 3, K=131072, N=3, LF flux, J = int_{r<1} rho^2 at step 200, checkpoint 20")
         return dict(name=name, mesh=mesh, params=params, cfl=0.5, desc=desc, m=4, analytic="vortex",
                     L=10.0)
+    if name == "c6":
+        n, N = 512, 4
+        mesh = E.structured_mesh(n, -10.0, 10.0, -10.0, 10.0, 0.15, 5, False)
+        params = dict(order=N, eqn=E.EQN_EULER5, flux=E.FLUX_HLL, bcs=[(E.BC_EULER_WALL, [])],
+                      limiter=(1.4, 1e-2, 1e-2))
+        desc = ("Hesthaven-Warburton Euler limiter (apl_limit, in place): Apollo's 5-variable "
+                "Euler, [-10,10]^2 with walls, 512x512x2 jittered 0.15h seed 5, K=524288, N=4, "
+                "isentropic-vortex state")
+        return dict(name=name, mesh=mesh, params=params, cfl=0.5, desc=desc, m=5, analytic="vortex5",
+                    L=20.0)
     raise SystemExit("unknown workload %s" % name)
 
 
@@ -286,6 +296,46 @@ def run_leg(name, steps, variants, torch):
     return out
 
 
+def bytes_alg_limit(K, Np, m):
+    """HBM bytes of one limiter application: read and write the state
+    (FP64) plus 140 B of geometry and connectivity per element (vertices,
+    J, three normals and half lengths, three face codes)."""
+    return K * (2 * Np * m * 8 + 140)
+
+
+def run_limiter(torch, reps=50):
+    """Time apl_limit in place on the c6 workload with HIP events on the
+    engine's stream; report applications/s, DOF/s and the algorithmic-byte
+    rate against the HBM peak."""
+    from apollo_amd import engine as E
+    w = workload("c6")
+    eng, stream = engine_on_stream(w, E.PREC_FP64, E.ALGO_AUTO, torch)
+    g, x, y = eng.geometry()
+    q0 = np.ascontiguousarray(E.vortex_state(x, y, 0.0, m=5))
+    with torch.cuda.stream(stream):
+        qd = torch.from_numpy(q0.copy()).cuda()
+        for _ in range(3):
+            eng.limit(qd, qd)
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(reps):
+            eng.limit(qd, qd)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+    sec = ev0.elapsed_time(ev1) * 1e-3 / reps
+    K, Np, m = eng.K, eng.Np, eng.m
+    eng.close()
+    ach = bytes_alg_limit(K, Np, m) / sec
+    return {"workload": w["desc"], "reps": reps, "us_per_application": sec * 1e6,
+            "dof_per_s": K * Np * m / sec,
+            "roofline": {"bound": "hbm", "achieved": ach / 1e9, "peak": PEAK["hbm"] / 1e9,
+                         "unit": "GB/s", "frac": ach / PEAK["hbm"],
+                         "bytes_alg_per_element": bytes_alg_limit(1, Np, m)},
+            "note": "time of apl_limit includes its host-side launch of three kernels and one "
+                    "flag check per call"}
+
+
 def cpu_baseline(w, nthreads, budget_s=15.0):
     """Oracle RHS (Apollo's loop restated in C, Apollo's 8-variable Maxwell /
     5-variable Euler layout) on the host cores, on the full mesh."""
@@ -369,6 +419,9 @@ def main():
     import torch
     from apollo_amd import engine as E
     assert torch.cuda.is_available()
+    if args.workload == "c6":
+        print(json.dumps({"limiter_c6": run_limiter(torch)}), flush=True)
+        return
     w = workload(args.workload)
     linear_ok = w["m"] == 3
     res = {}
@@ -472,6 +525,8 @@ def main():
                  ("bf16x3", E.PREC_BF16X3, E.ALGO_AUTO)]
         line["euler_c3"] = run_leg("c3", 200, three, torch)
         line["maxwell_c5"] = run_leg("c5", 300, three, torch)
+        print("bench: limiter (c6)", file=sys.stderr, flush=True)
+        line["limiter_c6"] = run_limiter(torch)
     if not args.no_cpu_baseline:
         nth = min(16, os.cpu_count() or 1)
         print("bench: cpu baseline, %d threads" % nth, file=sys.stderr, flush=True)
