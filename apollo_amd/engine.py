@@ -377,6 +377,23 @@ class Engine:
     def suggested_dt(self, cfl, max_speed):
         return 2.0 / 3.0 * cfl * self.dtscale * (self.rmin / max_speed)
 
+    def solve(self, t_end, nout, dt, cfl, q, run_name=None, mesh=None, stages=5):
+        """ApSolver::solve (src/solvers/apsolver.cc:172-216): write frame 0,
+        then nout frames of length t_end/nout, each TSSolve restarted from
+        dt, writing `<run_name>_<i>.vtu` after each when run_name is given
+        (needs `mesh`).  q (host) is advanced in place; returns the number
+        of steps of each frame."""
+        from . import vtu
+        if run_name is not None:
+            vtu.write_frame(vtu.frame_name(run_name, 0), mesh, q)
+        tsize = t_end / nout
+        steps = []
+        for i in range(nout):
+            steps.append(len(self.ssp_advance(i * tsize, (i + 1) * tsize, dt, cfl, q, stages=stages)))
+            if run_name is not None:
+                vtu.write_frame(vtu.frame_name(run_name, i + 1), mesh, q)
+        return steps
+
 
 # ---------------------------------------------------------------------------
 # initial conditions of the benchmark configurations (closed forms)

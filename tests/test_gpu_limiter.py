@@ -133,3 +133,24 @@ def test_lserk4_with_limiter_matches_oracle():
     err = rel_l2(q - q0, qo - q0)
     print("LSERK4 + limiter increment rel L2 %.3e" % err)
     assert err <= 1e-11
+
+
+def test_solve_writes_apollo_frames(tmp_path):
+    """ApSolver::solve on the forward-facing-step deck: frame 0 is the
+    initial state, frames 1..nout follow each TSSolve; the last frame holds
+    the engine's state."""
+    from apollo_amd import vtu
+    case = ffs_case(N=1)
+    q = ffs_state(case)
+    eng = E.Engine(case.mesh, case.params)
+    dt0 = eng.initial_dt(q, 1.0 / 3.0)
+    run = str(tmp_path / "ffstep")
+    steps = eng.solve(6 * dt0, 2, dt0, 1.0 / 3.0, q, run_name=run, mesh=case.mesh)
+    eng.close()
+    assert len(steps) == 2 and min(steps) >= 3
+    pts, cells, arrays = vtu.read_frame(vtu.frame_name(run, 2))
+    m = case.params.m
+    got = np.stack([arrays[1 + j][1] for j in range(case.ops.Np * m)], -1).reshape(q.shape)
+    assert np.array_equal(got, q)
+    _, _, a0 = vtu.read_frame(vtu.frame_name(run, 0))
+    assert np.allclose(a0[1][1], ffs_state(case)[:, 0, 0])

@@ -203,3 +203,34 @@ def test_cpp_host_interface_compiles(tmp_path):
     subprocess.check_call(["g++", "-O1", "-std=c++17", "-Wall", "-Werror", "-o", str(exe), src,
                            "-L" + os.path.join(ROOT, "apollo_amd"), "-lapollo_dg"])
     assert exe.exists()
+
+
+# --- per-frame VTU output ---------------------------------------------------------
+
+def test_vtu_frame_layout(tmp_path):
+    """The layout Apollo's readers expect (scripts/wxunsdgdata2.py:38-84):
+    cell array 0 is "Rank", array j + 1 is dof j = node * m + variable, the
+    cells' first three points are the element corners, and the nodes
+    rebuilt from them with Apollo's affine map are the engine's nodes."""
+    from apollo_amd import vtu
+    from apollo_amd.gmsh import read_gmsh
+    mesh = read_gmsh(os.path.join(GOLDEN, "meshes", "ffs_grid.msh"))
+    N, m = 3, 5
+    ops = O.Ops(N)
+    x, y, _ = O.node_coords(ops, mesh.VX, mesh.VY, mesh.EToV)
+    rng = np.random.default_rng(0)
+    q = rng.standard_normal((mesh.K, ops.Np, m))
+    path = tmp_path / vtu.frame_name("ffstep", 7)
+    assert path.name == "ffstep_7.vtu"
+    vtu.write_frame(str(path), mesh, q)
+    pts, cells, arrays = vtu.read_frame(str(path))
+    assert arrays[0][0] == "Rank" and len(arrays) == 1 + ops.Np * m
+    assert np.array_equal(cells, mesh.EToV)
+    for node in range(ops.Np):
+        for v in range(m):
+            assert np.array_equal(arrays[m * node + v + 1][1], q[:, node, v])
+    p1, p2, p3 = pts[cells[:, 0]], pts[cells[:, 1]], pts[cells[:, 2]]
+    r, s = ops.r[None, :], ops.s[None, :]
+    X = 0.5 * (-p1[:, :1] * (r + s) + p2[:, :1] * (1. + r) + p3[:, :1] * (1. + s))
+    Y = 0.5 * (-p1[:, 1:2] * (r + s) + p2[:, 1:2] * (1. + r) + p3[:, 1:2] * (1. + s))
+    assert np.abs(X - x).max() < 1e-14 and np.abs(Y - y).max() < 1e-14
