@@ -814,13 +814,21 @@ __device__ __forceinline__ void split_state(const double *XT, int lg, int el, bf
         split_bf16(float(XT[((32 * kk + 4 * it + lg) * M + v) * XS + el]), qh[kk][v], ql[kk][v], it);
 }
 
-// Waves per SIMD the register allocator is asked to allow: two where the
-// kernel fits 256 VGPRs without spilling (measured with
-// -Rpass-analysis=kernel-resource-usage), one otherwise.
-template <int N, int EQN>
+// Waves per SIMD the register allocator is asked to allow, by measurement
+// (K = 131072, stage time one wave -> two): two for N <= 4; at N = 5 for
+// Maxwell and for FP32 Euler (303.6 -> 257.4 us; FP64 Euler spills and
+// slows, 346 -> 373); at N = 6 for Maxwell (FP64 303 -> 293, FP32 246 ->
+// 226; Euler no gain); one for N >= 7, where two waves spill (FP64) or
+// measured slower (FP32 Euler N = 8: 843 -> 886 us).
+template <int N, int EQN, typename T>
 struct MinWaves {
+  static constexpr bool TM = EQN == APL_EQN_MAXWELL_TM3;
   static constexpr int value =
-      (EQN == APL_EQN_PHMAXWELL8 || N >= 6) ? 1 : (N == 5 ? (EQN == APL_EQN_MAXWELL_TM3 ? 2 : 1) : 2);
+      EQN == APL_EQN_PHMAXWELL8 ? 1
+      : N <= 4                 ? 2
+      : N == 5                 ? ((TM || sizeof(T) == 4) ? 2 : 1)
+      : N == 6                 ? (TM ? 2 : 1)
+                               : 1;
 };
 
 // Split-BF16x3 cubature kernels stay at one wave per group: with the barriers
@@ -828,11 +836,11 @@ struct MinWaves {
 template <int N, int EQN, typename T>
 struct RhsGroup {
   static constexpr int W =
-      sizeof(T) == 1 ? 1 : group_waves(Lds<N, EQN, T>::doubles, 4 * MinWaves<N, EQN>::value);
+      sizeof(T) == 1 ? 1 : group_waves(Lds<N, EQN, T>::doubles, 4 * MinWaves<N, EQN, T>::value);
 };
 
 template <int N, int EQN, int FLUX, typename T>
-__global__ void __launch_bounds__((64 * RhsGroup<N, EQN, T>::W), (MinWaves<N, EQN>::value))
+__global__ void __launch_bounds__((64 * RhsGroup<N, EQN, T>::W), (MinWaves<N, EQN, T>::value))
     rhs_kernel(RhsArgs a) {
   using O = Order<N>;
   using L = Lds<N, EQN, T>;
@@ -1162,12 +1170,15 @@ __device__ __forceinline__ void gather_face_node(const double *q, int kf, int co
   }
 }
 
-// Waves per SIMD requested from the register allocator (measured with
-// -Rpass-analysis=kernel-resource-usage: no spills at these bounds).
+// Waves per SIMD requested from the register allocator.  FP64 runs two
+// waves at every order: at N = 7, 8 that costs 24-33 spilled VGPRs but beats
+// one wave with 380 registers (N = 8, K = 294912: 446 against 511 us per
+// stage; N = 7: 394 against 419).  FP32 and BF16x3 at three waves for
+// N >= 7 measured no faster than two.
 template <int N, int EQN, typename T>
 struct LinWaves {
   static constexpr int value = EQN == APL_EQN_PHMAXWELL8 ? 1
-                              : sizeof(T) == 8 ? (N <= 5 ? 2 : 1)
+                              : sizeof(T) == 8 ? 2
                                                : (N <= 5 ? 4 : (N == 6 ? 3 : 2));   // FP32, BF16x3
 };
 

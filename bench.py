@@ -358,10 +358,29 @@ def cpu_baseline(w, nthreads, budget_s=15.0):
         O.rhs(case.omesh, case.ophys, 0.0, q, nthreads=nthreads)
     el = time.perf_counter() - t0
     dofs = case.mesh.K * case.ops.Np * w["m"]
+    # one core: the same RHS, fewer repetitions
+    O.rhs(case.omesh, case.ophys, 0.0, q, nthreads=1)
+    reps1 = 3
+    t0 = time.perf_counter()
+    for _ in range(reps1):
+        O.rhs(case.omesh, case.ophys, 0.0, q, nthreads=1)
+    el1 = time.perf_counter() - t0
+    model = ""
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     return dict(value=dofs * reps / el, unit="DOF-updates/s", cores=nthreads, kind="port",
                 sample="%d RHS evaluations of the full mesh (K=%d, Apollo's %d-variable layout) "
                        "after 3 warm-ups, OpenMP over elements" % (reps, case.mesh.K, q.shape[-1]),
-                rhs_per_s=reps / el)
+                rhs_per_s=reps / el,
+                single_core={"value": dofs * reps1 / el1, "rhs_per_s": reps1 / el1,
+                             "sample": "%d RHS evaluations, 1 thread" % reps1},
+                host={"cpu_model": model, "nproc": os.cpu_count(), "usable_cores": usable})
 
 
 def measured_traffic(key):
