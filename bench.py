@@ -282,7 +282,10 @@ def run_leg(name, steps, variants, torch):
         lin = m == 3 and algo == 0
         d = dict(dof_updates_per_s=K * Np * m / kern_s, kernel_us=kern_s * 1e6,
                  form="nodal" if lin else "cubature",
-                 tflops_exec=flop_exec(lin, K, N, Np, Nc, Ng, m) / kern_s / 1e12)
+                 tflops_alg=flop_alg(K, N, Np, Nc, Ng, m) / kern_s / 1e12,
+                 tflops_exec=flop_exec(lin, K, N, Np, Nc, Ng, m) / kern_s / 1e12,
+                 gbs_alg=bytes_alg_stage(K, Np, m) / kern_s / 1e9)
+        d["hbm_frac_alg"] = d["gbs_alg"] * 1e9 / PEAK["hbm"]
         if vname != "fp64":
             d["l2_vs_fp64"] = float(np.linalg.norm((sel(r["qT"]) - sel(ref)).ravel())
                                     / np.linalg.norm(sel(ref).ravel()))
