@@ -25,6 +25,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "../../include/apollo_dg.h"
 
 namespace apl {
@@ -578,8 +580,8 @@ typedef double dbl2 __attribute__((ext_vector_type(2)));
 // XT[w*RS + e] = q[(e0 + e)*ES + w]: all loads of a full batch are issued
 // before any LDS write (16-byte vector loads; the batch base is 128-byte
 // aligned because 16*ES*8 is a multiple of 128).
-template <int ES, int RS = XS>
-__device__ __forceinline__ void stage_transposed(double *XT, const double *qb, int cnt, int lane) {
+template <int ES, int RS = XS, typename XE = double>
+__device__ __forceinline__ void stage_transposed(XE *XT, const double *qb, int cnt, int lane) {
   constexpr int NV2 = 16 * ES / 2;
   constexpr int NIT = (NV2 + 63) / 64;
   if (cnt == 16) {
@@ -596,14 +598,14 @@ __device__ __forceinline__ void stage_transposed(double *XT, const double *qb, i
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const int i = 2 * idx + h, e = i / ES, w = i - e * ES;
-          XT[w * RS + e] = buf[j][h];
+          XT[w * RS + e] = XE(buf[j][h]);
         }
       }
     }
   } else {
     for (int i = lane; i < 16 * ES; i += 64) {
       const int e = i / ES, w = i - e * ES;
-      XT[w * RS + e] = (e < cnt) ? qb[i] : qb[w];
+      XT[w * RS + e] = XE((e < cnt) ? qb[i] : qb[w]);
     }
   }
 }
@@ -612,8 +614,8 @@ __device__ __forceinline__ void stage_transposed(double *XT, const double *qb, i
 // stage update res = a res + dt rhs, q_out = q_in + b res (mode 1), or the
 // linear combination out = ca q + cb rhs (+ cz z) of an SSP stage (mode 2;
 // z may be `out` itself: each element reads its own z before writing).
-template <int ES>
-__device__ __forceinline__ unsigned write_batch(const RhsArgs &a, const double *OUT, long e0,
+template <int ES, typename OE = double>
+__device__ __forceinline__ unsigned write_batch(const RhsArgs &a, const OE *OUT, long e0,
                                                 int cnt, int lane) {
   constexpr int NV2 = 16 * ES / 2;
   constexpr int NIT = (NV2 + 63) / 64;
@@ -1141,8 +1143,16 @@ __global__ void __launch_bounds__((64 * RhsGroup<N, EQN, T>::W), (MinWaves<N, EQ
 template <int N, int EQN, typename T>
 struct LinLds {
   static constexpr int M = Vars<EQN>::M;
-  static constexpr int XT = xt_rows<N, T>() * M * XS;
-  static constexpr int doubles = XT;
+  // The staged state, which feeds the FP32 B operands and the face traces
+  // whose fluxes enter the FP32 products: FP32 (half the LDS) for BF16x3
+  // and for FP32 at N >= 6, where it measured faster (K = 294912, N = 8:
+  // FP32 371 -> 356 us per stage, BF16x3 385 -> 350; N = 5 BF16x3 68.4 ->
+  // 66.1); FP64 for the FP64 kernel and for FP32 at N <= 5, where the
+  // conversions cost registers at four waves per SIMD (64.7 -> 70.4 us).
+  using XE = typename std::conditional<sizeof(T) == 8 || (sizeof(T) == 4 && N <= 5), double,
+                                       float>::type;
+  static constexpr int XT = xt_rows<N, T>() * M * XS;          // elements of XE
+  static constexpr int doubles = (XT * (int)sizeof(XE) + 7) / 8;
 };
 
 template <int N>
@@ -1212,8 +1222,8 @@ __global__ void __launch_bounds__((64 * LinGroup<N, EQN, T>::W), (LinWaves<N, EQ
   const WaveBatch wb = wave_batch<W>(a.K);
   if (!wb.active) return;
   extern __shared__ double lds_all[];
-  double *lds = lds_all + wb.wave * L::doubles;
-  double *XT = lds;
+  using XE = typename L::XE;
+  XE *XT = (XE *)(lds_all + wb.wave * L::doubles);
 
   const int lane = wb.lane;
   const int el = lane & 15, lg = lane >> 4;
@@ -1248,8 +1258,8 @@ __global__ void __launch_bounds__((64 * LinGroup<N, EQN, T>::W), (LinWaves<N, EQ
       gather_face_node<Np, Nfp, M>(qg, 4 * j + lg, code0, code1, code2, fnb[j], qP[j]);
   }
 
-  stage_transposed<ES>(XT, a.q + e0 * ES, cnt, lane);
-  for (int i = ES * XS + lane; i < L::XT; i += 64) XT[i] = 0.0;
+  stage_transposed<ES, XS, XE>(XT, a.q + e0 * ES, cnt, lane);
+  for (int i = ES * XS + lane; i < L::XT; i += 64) XT[i] = XE(0);
   wave_sync();
 
   unsigned ef = 0;
@@ -1284,7 +1294,7 @@ __global__ void __launch_bounds__((64 * LinGroup<N, EQN, T>::W), (LinWaves<N, EQ
       double *qp = qP[PRE ? ks : 0];
       if constexpr (!PRE) gather_face_node<Np, Nfp, M>(qg, kf, code0, code1, code2, fnb[ks], qp);
 #pragma unroll
-      for (int v = 0; v < M; ++v) qM[v] = XT[(fown[ks] * M + v) * XS + el];
+      for (int v = 0; v < M; ++v) qM[v] = double(XT[(fown[ks] * M + v) * XS + el]);
       if (code < 0) boundary_state<EQN>(a.bc, -code - 1, 0.0, 0.0, 0.0, nx, ny, qM, qp);
       double lam;
       ef |= num_flux<EQN, FLUX>(a.ph, nx, ny, qM, qp, fs, &lam);
@@ -1387,7 +1397,7 @@ __global__ void __launch_bounds__((64 * LinGroup<N, EQN, T>::W), (LinWaves<N, EQ
   // epilogue: / J, stage through LDS as [e][n][v], coalesced write
   const double Jel = gk[G_J];
   wave_sync();
-  double *OUT = lds;
+  XE *OUT = XT;
 #pragma unroll
   for (int rt = 0; rt < NRT; ++rt)
 #pragma unroll
@@ -1395,7 +1405,7 @@ __global__ void __launch_bounds__((64 * LinGroup<N, EQN, T>::W), (LinWaves<N, EQ
       const int n = 16 * rt + acc_row<T>(lg, r);
       if (n < Np) {
 #pragma unroll
-        for (int v = 0; v < M; ++v) OUT[el * ES + n * M + v] = double(acc[rt][v][r]) / Jel;
+        for (int v = 0; v < M; ++v) OUT[el * ES + n * M + v] = XE(double(acc[rt][v][r]) / Jel);
       }
     }
   wave_sync();
