@@ -722,20 +722,30 @@ template <int N, int EQN, typename T>
 struct Lds {
   static constexpr int M = Vars<EQN>::M;
   static constexpr int Np = Order<N>::Np, Nfp = Order<N>::Nfp, Ng = Order<N>::Ng;
-  static constexpr int XT = xt_rows<N, T>() * M * XS;   // rows >= Np are zero
-  static constexpr int PT = 3 * Nfp * M * 16;
+  // the staged state and neighbour face nodes: FP32 for the FP32 and
+  // BF16x3 kernels (half the LDS, so more waves fit a CU), FP64 for FP64
+  using XE = typename std::conditional<sizeof(T) == 8, double, float>::type;
+  static constexpr int XT = xt_rows<N, T>() * M * XS;   // elements; rows >= Np are zero
+  static constexpr int PT = 3 * Nfp * M * 16;           // elements
+  static constexpr int XT_D = (XT * (int)sizeof(XE) + 7) / 8;   // in doubles
+  static constexpr int PT_D = (PT * (int)sizeof(XE) + 7) / 8;
   static constexpr int I1D = 3 * Ng * Nfp;
   static constexpr int FM = (3 * Nfp + 1) / 2;   // ints packed into doubles
-  static constexpr int doubles = XT + PT + I1D + FM;
+  static constexpr int TABLES = XT_D + PT_D + I1D + FM;
+  static constexpr int OUT = 16 * Np * M;        // FP64 result staging, over the same space
+  static constexpr int doubles = TABLES > OUT ? TABLES : OUT;
 };
 
-// Per-lane state of the cubature kernel's surface pass.
+// Per-lane state of the cubature kernel's surface pass (XE: the staged
+// state's element type).
+template <typename XE>
 struct FaceCtx {
   int lg, el;
   long ge;
   int code[3];
   double nx[3], ny[3], sj[3];
-  const double *XT, *PT, *I1;
+  const XE *XT, *PT;
+  const double *I1;
   const int *FM;
   double tnow, lmax;
   unsigned ef;
@@ -744,8 +754,8 @@ struct FaceCtx {
 // Scaled numerical flux fsT = F* L/2 at face Gauss point kf = 4 ks + lg of
 // element el (zero past the last point): traces from the staged own and
 // neighbour face nodes, the exterior state on boundary faces.
-template <int N, int EQN, int FLUX, typename S>
-__device__ __forceinline__ void face_flux_cub(const RhsArgs &a, int ks, S *fsT, FaceCtx &F) {
+template <int N, int EQN, int FLUX, typename S, typename XE>
+__device__ __forceinline__ void face_flux_cub(const RhsArgs &a, int ks, S *fsT, FaceCtx<XE> &F) {
   constexpr int M = Vars<EQN>::M;
   constexpr int Ng = Order<N>::Ng, Nfp = Order<N>::Nfp;
   const int kf = 4 * ks + F.lg;
@@ -767,9 +777,9 @@ __device__ __forceinline__ void face_flux_cub(const RhsArgs &a, int ks, S *fsT, 
 #pragma unroll
     for (int i = 0; i < Nfp; ++i) {
       const double w = Io[i];
-      const double *xo = F.XT + F.FM[f * Nfp + i] * M * XS + F.el;
+      const XE *xo = F.XT + F.FM[f * Nfp + i] * M * XS + F.el;
 #pragma unroll
-      for (int v = 0; v < M; ++v) qM[v] += w * xo[v * XS];
+      for (int v = 0; v < M; ++v) qM[v] += w * double(xo[v * XS]);
     }
     if (code >= 0) {
       // own node i <-> neighbour node Fmask[nf][Nfp-1-i]; the neighbour's
@@ -780,7 +790,7 @@ __device__ __forceinline__ void face_flux_cub(const RhsArgs &a, int ks, S *fsT, 
       for (int i = 0; i < Nfp; ++i) {
         const double w = In[Nfp - 1 - i];
 #pragma unroll
-        for (int v = 0; v < M; ++v) qP[v] += w * F.PT[((f * Nfp + i) * M + v) * 16 + F.el];
+        for (int v = 0; v < M; ++v) qP[v] += w * double(F.PT[((f * Nfp + i) * M + v) * 16 + F.el]);
       }
     } else {
       const double *vk = a.vert + F.ge * 6;
@@ -804,8 +814,8 @@ __device__ __forceinline__ void face_flux_cub(const RhsArgs &a, int ks, S *fsT, 
 
 // BF16x3 B operands of a K = Np product: row 32 kk + 4 it + lg of the staged
 // state (element el) split into hi/lo item it of step kk.
-template <int NK8, int M>
-__device__ __forceinline__ void split_state(const double *XT, int lg, int el, bf8 (&qh)[NK8][M],
+template <int NK8, int M, typename XE>
+__device__ __forceinline__ void split_state(const XE *XT, int lg, int el, bf8 (&qh)[NK8][M],
                                             bf8 (&ql)[NK8][M]) {
 #pragma unroll
   for (int kk = 0; kk < NK8; ++kk)
@@ -862,9 +872,10 @@ __global__ void __launch_bounds__((64 * RhsGroup<N, EQN, T>::W), (MinWaves<N, EQ
   if (!wb.active) return;
   extern __shared__ double lds_all[];
   double *lds = lds_all + wb.wave * L::doubles;
-  double *XT = lds;
-  double *PT = lds + L::XT;
-  double *I1 = PT + L::PT;
+  using XE = typename L::XE;
+  XE *XT = (XE *)lds;
+  XE *PT = (XE *)(lds + L::XT_D);
+  double *I1 = lds + L::XT_D + L::PT_D;
   int *FM = (int *)(I1 + L::I1D);
 
   const int lane = wb.lane;
@@ -887,8 +898,8 @@ __global__ void __launch_bounds__((64 * RhsGroup<N, EQN, T>::W), (MinWaves<N, EQ
   for (int i = lane; i < 3 * Nfp; i += 64) FM[i] = a.fmask[i];
 
   // 2. stage the batch state, transposed (coalesced global reads)
-  stage_transposed<ES>(XT, a.q + e0 * ES, cnt, lane);
-  for (int i = ES * XS + lane; i < L::XT; i += 64) XT[i] = 0.0;
+  stage_transposed<ES, XS, XE>(XT, a.q + e0 * ES, cnt, lane);
+  for (int i = ES * XS + lane; i < L::XT; i += 64) XT[i] = XE(0);
   wave_sync();
 
   // 3. gather the neighbours' face nodes; item (k = f*Nfp + i, e), e fastest
@@ -912,7 +923,7 @@ __global__ void __launch_bounds__((64 * RhsGroup<N, EQN, T>::W), (MinWaves<N, EQ
         for (int v = 0; v < M; ++v) v8[v] = 0.0;
       }
 #pragma unroll
-      for (int v = 0; v < M; ++v) PT[(k * M + v) * 16 + e] = v8[v];
+      for (int v = 0; v < M; ++v) PT[(k * M + v) * 16 + e] = XE(v8[v]);
     }
   }
   wave_sync();
@@ -1044,7 +1055,7 @@ __global__ void __launch_bounds__((64 * RhsGroup<N, EQN, T>::W), (MinWaves<N, EQ
   }
 
   // 5. surface: face Gauss point kf = 4 ks + lg of element el
-  FaceCtx F;
+  FaceCtx<XE> F;
   F.lg = lg;
   F.el = el;
   F.ge = ge;
