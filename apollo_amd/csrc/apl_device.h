@@ -827,19 +827,22 @@ __device__ __forceinline__ void split_state(const XE *XT, int lg, int el, bf8 (&
 }
 
 // Waves per SIMD the register allocator is asked to allow, by measurement
-// (K = 131072, stage time one wave -> two): two for N <= 4; at N = 5 for
-// Maxwell and for FP32 Euler (303.6 -> 257.4 us; FP64 Euler spills and
-// slows, 346 -> 373); at N = 6 for Maxwell (FP64 303 -> 293, FP32 246 ->
-// 226; Euler no gain); one for N >= 7, where two waves spill (FP64) or
-// measured slower (FP32 Euler N = 8: 843 -> 886 us).
+// (K = 131072, stage time one wave -> two):
+//  - two for N <= 4, and for FP32 at every order (with the FP32-staged
+//    state: Euler N = 7 515 -> 383 us, N = 8 616 -> 594);
+//  - BF16x3: two for Maxwell at every order (N = 7 315 -> 234, N = 8
+//    371 -> 276), one for Euler above N = 4 (N = 8 565 -> 739 with two);
+//  - FP64: two for Maxwell up to N = 6 (N = 6 303 -> 293), one for Euler
+//    above N = 4 and for N >= 7, where two waves spill (Euler N = 5
+//    346 -> 373, N = 7 637 -> 1380).
 template <int N, int EQN, typename T>
 struct MinWaves {
   static constexpr bool TM = EQN == APL_EQN_MAXWELL_TM3;
   static constexpr int value =
       EQN == APL_EQN_PHMAXWELL8 ? 1
-      : N <= 4                 ? 2
-      : N == 5                 ? ((TM || sizeof(T) == 4) ? 2 : 1)
-      : N == 6                 ? (TM ? 2 : 1)
+      : (N <= 4 || sizeof(T) == 4) ? 2
+      : sizeof(T) == 1         ? (TM ? 2 : 1)
+      : N <= 6                 ? (TM ? 2 : 1)
                                : 1;
 };
 
