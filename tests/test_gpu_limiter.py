@@ -154,3 +154,47 @@ def test_solve_writes_apollo_frames(tmp_path):
     assert np.array_equal(got, q)
     _, _, a0 = vtu.read_frame(vtu.frame_name(run, 0))
     assert np.allclose(a0[1][1], ffs_state(case)[:, 0, 0])
+
+
+def test_cpp_host_runs_the_ffs_deck(tmp_path):
+    """The C++ host (apollo_amd/host: the Gmsh reader with DMPlex's
+    numbering and the Apollo-shaped subsolver) runs the forward-facing-step
+    deck: step() limits its input in place and returns the RHS; solve() runs
+    two frames.  The results equal the Python binding's bit for bit (same
+    library, same sequence of launches) and the oracle to the FP64 bar."""
+    import os
+    import subprocess
+    from cases import GOLDEN
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "ffs_deck_demo")
+    assert os.path.exists(exe), "build tests/cpp first (make -C tests/cpp)"
+    out = tmp_path / "ffs.bin"
+    r = subprocess.run([exe, os.path.join(GOLDEN, "meshes", "ffs_grid.msh"), "1", "2", "6.0",
+                        str(out)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    raw = np.fromfile(str(out), dtype=np.float64)
+    dt0, sdt, n = raw[0], raw[1], int(raw[2])
+    q0, qs, rhs, qf = (raw[3 + i * n:3 + (i + 1) * n] for i in range(4))
+    case = ffs_case(N=1)
+    shape = case.shape
+    q0, qs, rhs, qf = (a.reshape(shape) for a in (q0, qs, rhs, qf))
+    # the deck's state (the C++ expression orders the products differently)
+    assert np.abs(q0 - ffs_state(case)).max() <= 1e-14 * np.abs(q0).max()
+    eng = E.Engine(case.mesh, case.params)
+    cfl = 1.0 / 3.0
+    assert dt0 == eng.initial_dt(q0, cfl)
+    ql = np.zeros_like(q0)
+    eng.limit(q0, ql)
+    assert np.array_equal(qs, ql)
+    r2 = np.zeros_like(q0)
+    assert eng.rhs(0.0, ql, r2, cfl=cfl) == sdt
+    assert np.array_equal(rhs, r2)
+    qe = q0.copy()
+    eng.solve(6.0 * dt0, 2, dt0, cfl, qe)
+    eng.close()
+    assert np.array_equal(qf, qe)
+    # against the oracle: the limiter and the RHS of the limited state
+    qo = to_oracle(case.params.eqn, q0)
+    _close(qs, from_oracle(case.params.eqn, O.limit(case.omesh, case.ophys, qo)))
+    from cases import fp64_parity
+    fp64_parity(case, rhs, qs)
