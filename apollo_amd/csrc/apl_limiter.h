@@ -69,9 +69,19 @@ template <int ES>
 __device__ __forceinline__ void lim_stage_in(const double *src, double *lds, int cnt, int lane) {
   const int n = cnt * ES;
   if (cnt == LE) {
-    constexpr int NV2 = LE * ES / 2;
-#pragma unroll 4
-    for (int i = lane; i < NV2; i += 64) ((dbl2 *)lds)[i] = ((const dbl2 *)src)[i];
+    // every load of the wave in flight before the first LDS write
+    constexpr int NV2 = LE * ES / 2, NIT = (NV2 + 63) / 64;
+    dbl2 buf[NIT];
+#pragma unroll
+    for (int j = 0; j < NIT; ++j) {
+      const int i = lane + 64 * j;
+      if (i < NV2) buf[j] = ((const dbl2 *)src)[i];
+    }
+#pragma unroll
+    for (int j = 0; j < NIT; ++j) {
+      const int i = lane + 64 * j;
+      if (i < NV2) ((dbl2 *)lds)[i] = buf[j];
+    }
   } else {
     for (int i = lane; i < n; i += 64) lds[i] = src[i];
   }
