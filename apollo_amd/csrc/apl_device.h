@@ -227,17 +227,36 @@ __device__ __forceinline__ unsigned phys_flux(const PhysParams &P, const S *q, S
   return ef;
 }
 
-// LF wave speed |u| + c (wxeulereqn.cc:1106-1110, primitive variables
-// :1324-1341).
+// Euler flux of one trace and its LF wave speed |u| + c
+// (wxeulereqn.cc:806-871, 1106-1110) from one reciprocal of the density and
+// one pressure; the density and pressure floors act as in phys_flux.
 template <int EQN>
-__device__ __forceinline__ double euler_speed(const PhysParams &P, const double *q) {
+__device__ __forceinline__ unsigned euler_flux_speed(const PhysParams &P, const double *q,
+                                                     double *F, double *G, double *speed) {
   constexpr bool W = (EQN == APL_EQN_EULER5);
   constexpr int IE = W ? 4 : 3;
-  const double ir = 1.0 / q[0];
-  const double mw = W ? q[3] : 0.0;
-  const double p = (P.gamma - 1.) * (q[IE] - 0.5 * (q[1] * q[1] + q[2] * q[2] + mw * mw) * ir);
-  const double u = q[1] * ir, v = q[2] * ir, w = mw * ir;
-  return sqrt(u * u + v * v + w * w) + sqrt(P.gamma * p * ir);
+  unsigned ef = 0;
+  double rho = q[0];
+  if (P.min_dens == 0.0 && !(rho > 0.0)) ef |= EF_DENSITY;
+  if (rho < P.min_dens) rho = P.min_dens;
+  const double ir = 1.0 / rho;
+  const double u = q[1] * ir, v = q[2] * ir, w = W ? q[3] * ir : 0.0;
+  const double E = q[IE];
+  const double k2 = u * u + v * v + w * w;
+  double p = (P.gamma - 1.0) * (E - 0.5 * rho * k2);
+  if (!(p > 0.0)) {
+    if (P.min_pres == 0.0) ef |= EF_PRESSURE;
+    else p = P.min_pres;
+  }
+  const double ru = rho * u, rv = rho * v;
+  F[0] = ru;         G[0] = rv;
+  F[1] = ru * u + p; G[1] = rv * u;
+  F[2] = ru * v;     G[2] = rv * v + p;
+  if constexpr (W) { F[3] = ru * w; G[3] = rv * w; }
+  F[IE] = (E + p) * u;
+  G[IE] = (E + p) * v;
+  *speed = sqrt(k2) + sqrt(P.gamma * p * ir);
+  return ef;
 }
 
 // HLL (wxeulereqn.cc:1119-1202) and Roe (:1204-1268) fluxes of the Euler
@@ -348,13 +367,17 @@ __device__ __forceinline__ unsigned num_flux(const PhysParams &P, double nx, dou
     return num_flux<EQN, APL_FLUX_LF>(P, nx, ny, rM, rP, fs, lam);
   }
   double FM[M], GM[M], FP[M], GP[M];
-  unsigned ef = phys_flux<EQN, double>(P, qM, FM, GM);
-  ef |= phys_flux<EQN, double>(P, qP, FP, GP);
+  unsigned ef = 0;
   double l;
   if constexpr (EQN == APL_EQN_EULER5 || EQN == APL_EQN_EULER4) {
-    const double cM = euler_speed<EQN>(P, qM), cP = euler_speed<EQN>(P, qP);
+    // flux and |u| + c of each trace from one reciprocal and one pressure
+    double cM, cP;
+    ef |= euler_flux_speed<EQN>(P, qM, FM, GM, &cM);
+    ef |= euler_flux_speed<EQN>(P, qP, FP, GP, &cP);
     l = cM > cP ? cM : cP;
   } else {
+    ef |= phys_flux<EQN, double>(P, qM, FM, GM);
+    ef |= phys_flux<EQN, double>(P, qP, FP, GP);
     const double a = P.chi * P.c0, b = P.gph * P.c0;
     const double m1 = a > b ? a : b;
     l = m1 > P.c0 ? m1 : P.c0;
