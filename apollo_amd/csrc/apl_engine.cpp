@@ -608,8 +608,8 @@ int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out)
       return bail(APL_ERR_UNSUPPORTED);
     }
     // ave[Np] = 0.5 sum_m Mass[m][k], Mass = (V V^T)^-1 (wxnodaldggeometry2d.h:84-98),
-    // then r, s and drop[i][j] = delta_ij - ave[j]
-    std::vector<double> mass((size_t)Np * Np), tab((size_t)3 * Np + (size_t)Np * Np);
+    // then the reference-triangle node coordinates r, s
+    std::vector<double> mass((size_t)Np * Np), tab((size_t)3 * Np);
     if (!invert(Np, o.VVT.data(), mass.data())) {
       h->last_error = "mass matrix inversion failed";
       return bail(APL_ERR_ORDER);
@@ -621,8 +621,6 @@ int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out)
       tab[Np + k] = o.r[k];
       tab[2 * Np + k] = o.s[k];
     }
-    for (int i = 0; i < Np; ++i)
-      for (int j = 0; j < Np; ++j) tab[3 * Np + (size_t)i * Np + j] = -tab[j] + (i == j ? 1. : 0.);
     if ((rc = dev_upload<double>(h, &h->d_ltab, tab.data(), tab.size()))) return bail(rc);
     if ((rc = dev_alloc(h, (void **)&h->d_lavg, sizeof(double) * 8 * (size_t)h->K))) return bail(rc);
     if ((rc = dev_alloc(h, (void **)&h->d_lgrad, sizeof(double) * 40 * (size_t)h->K))) return bail(rc);
