@@ -66,8 +66,10 @@ struct apl_handle_s {
   double *d_ckpt = nullptr;
   long long ckpt_slots = 0;
   double *d_opart = nullptr;
-  // limiter: cell averages, gradients, weight and node tables
-  double *d_lavg = nullptr, *d_lgrad = nullptr, *d_ltab = nullptr;
+  // limiter: cell averages, gradients, weight and node tables, and the
+  // states at the trace nodes
+  double *d_lavg = nullptr, *d_lgrad = nullptr, *d_ltab = nullptr, *d_ltrc = nullptr;
+  LimTrace ltr{};
 };
 
 // ---------------------------------------------------------------------------
@@ -390,7 +392,7 @@ static void free_handle(apl_handle *h) {
   void *bufs[] = {h->d_geom, h->d_vert, h->d_I1D, h->d_fcode, h->d_fmask, h->d_A1, h->d_A2,
                   h->d_A3, h->d_err, h->d_speed, h->d_t, h->d_qa, h->d_qb, h->d_res, h->d_out,
                   h->d_ckpt, h->d_Vc, h->d_cub, h->d_sbar, h->d_qbar, h->d_rbar, h->d_stg,
-                  h->d_seg, h->d_opart, h->d_lavg, h->d_lgrad, h->d_ltab};
+                  h->d_seg, h->d_opart, h->d_lavg, h->d_lgrad, h->d_ltab, h->d_ltrc};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (double *b : h->d_adjops)
@@ -607,23 +609,55 @@ int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out)
       h->last_error = "limiter: face node lists do not chain at the vertices";
       return bail(APL_ERR_UNSUPPORTED);
     }
+    // trace slots: the nodes the gradient pass reads at the three face ends
+    // p = Fmask[0][0], Fmask[0][Nfp-1], Fmask[1][Nfp-1], each taken as the
+    // face-node position (face p / Nfp, position j = p % Nfp): its own node
+    // Fmask[p], and node Nfp-1-j of whichever face nf a neighbour shares
+    LimTrace &tr = h->ltr;
+    tr.s = 0;
+    auto slot = [&tr](int node) {
+      for (int q = 0; q < tr.s; ++q)
+        if (tr.node[q] == node) return q;
+      tr.node[tr.s] = node;
+      return tr.s++;
+    };
+    const int pe[3] = {0, Nfp - 1, 2 * Nfp - 1};
+    for (int i = 0; i < 3; ++i) {
+      const int p = F[pe[i]];
+      if (p >= 3 * Nfp) {
+        h->last_error = "limiter: a face end falls outside the face node lists";
+        return bail(APL_ERR_UNSUPPORTED);
+      }
+      const int fe = p / Nfp, j = p - fe * Nfp;
+      tr.face[i] = fe;
+      tr.own[i] = slot(F[p]);
+      for (int nf = 0; nf < 3; ++nf) tr.nbr[i][nf] = slot(F[nf * Nfp + Nfp - 1 - j]);
+    }
     // ave[Np] = 0.5 sum_m Mass[m][k], Mass = (V V^T)^-1 (wxnodaldggeometry2d.h:84-98),
-    // then the reference-triangle node coordinates r, s
+    // then the centred reference coordinates r_k - sum_j ave_j r_j and
+    // s_k - sum_j ave_j s_j
     std::vector<double> mass((size_t)Np * Np), tab((size_t)3 * Np);
     if (!invert(Np, o.VVT.data(), mass.data())) {
       h->last_error = "mass matrix inversion failed";
       return bail(APL_ERR_ORDER);
     }
+    double rbar = 0.0, sbar = 0.0;
     for (int k = 0; k < Np; ++k) {
       double a = 0.0;
       for (int m = 0; m < Np; ++m) a += 0.5 * mass[(size_t)k + (size_t)m * Np];
       tab[k] = a;
-      tab[Np + k] = o.r[k];
-      tab[2 * Np + k] = o.s[k];
+      rbar += a * o.r[k];
+      sbar += a * o.s[k];
+    }
+    for (int k = 0; k < Np; ++k) {
+      tab[Np + k] = o.r[k] - rbar;
+      tab[2 * Np + k] = o.s[k] - sbar;
     }
     if ((rc = dev_upload<double>(h, &h->d_ltab, tab.data(), tab.size()))) return bail(rc);
     if ((rc = dev_alloc(h, (void **)&h->d_lavg, sizeof(double) * 8 * (size_t)h->K))) return bail(rc);
     if ((rc = dev_alloc(h, (void **)&h->d_lgrad, sizeof(double) * 40 * (size_t)h->K))) return bail(rc);
+    if ((rc = dev_alloc(h, (void **)&h->d_ltrc, sizeof(double) * 5 * (size_t)tr.s * h->K)))
+      return bail(rc);
   }
   *out = h;
   return APL_OK;
@@ -640,7 +674,8 @@ static int enqueue_limit(apl_handle *h, const double *qin, double *qout) {
   a.geom = h->d_geom;
   a.vert = h->d_vert;
   a.fcode = h->d_fcode;
-  a.fmask = h->d_fmask;
+  a.trc = h->d_ltrc;
+  a.tr = h->ltr;
   a.tab = h->d_ltab;
   a.gamma = h->params.lim_gas_gamma;
   a.min_dens = h->params.lim_min_density;

@@ -118,10 +118,11 @@ template int launch_objective<APL_ORDER>(int, hipStream_t, const ObjArgs &, int)
 
 template <int N, int EQN>
 static int launch_limiter_typed(hipStream_t s, const LimArgs &a) {
-  const dim3 grid((a.K + 255) / 256), block(256), wgrid((a.K + LE - 1) / LE), wave(64);
+  const dim3 grid((a.K + 255) / 256), block(256), wgrid((a.K + LE - 1) / LE),
+      rgrid((a.K + LER - 1) / LER), wave(64);
   hipLaunchKernelGGL((lim_average_kernel<N, EQN>), wgrid, wave, 0, s, a);
   hipLaunchKernelGGL((lim_gradient_kernel<N, EQN>), grid, block, 0, s, a);
-  hipLaunchKernelGGL((lim_reconstruct_kernel<N, EQN>), wgrid, wave, 0, s, a);
+  hipLaunchKernelGGL((lim_reconstruct_kernel<N, EQN>), rgrid, wave, 0, s, a);
   return hipGetLastError() == hipSuccess ? APL_OK : APL_ERR_HIP;
 }
 

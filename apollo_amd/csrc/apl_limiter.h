@@ -4,7 +4,10 @@
 //
 //   lim_average_kernel      cell averages of the conserved variables
 //                           (computeConservedAndPrimitiveAVEVariables,
-//                           :487-522), one FP64 5-vector per element;
+//                           :487-522), one FP64 5-vector per element, and
+//                           the states at the few nodes the gradient pass
+//                           reads (LimTrace), so that it gathers from a
+//                           compact array instead of the full state;
 //   lim_gradient_kernel     the four-element patch of each element: patch
 //                           centres and areas, ghost averages through the
 //                           boundary states, face averages at the six face
@@ -33,6 +36,21 @@
 
 namespace apl {
 
+// The element nodes whose states the gradient pass reads, as slots of a
+// per-element trace the average pass writes.  End i of the three face ends
+// lies on face `face[i]`; its own state is slot `own[i]` of the element (or,
+// on a boundary face, of the last neighbour visited), and the neighbour
+// sharing its face nf contributes slot `nbr[i][nf]`.  Built on the host from
+// Fmask: the three vertices for N = 1, 2, 4, seven nodes for N = 3.
+constexpr int LIM_MAX_TR = 12;
+struct LimTrace {
+  int s;
+  int node[LIM_MAX_TR];
+  int face[3];
+  int own[3];
+  int nbr[3][3];
+};
+
 struct LimArgs {
   int K;
   const double *qin;       // [K][Np][M]
@@ -47,8 +65,10 @@ struct LimArgs {
   const double *geom;      // [K][G_WORDS]
   const double *vert;      // [K][6]
   const int *fcode;        // [K][3]
-  const int *fmask;        // [3][Nfp]
-  const double *tab;       // ave[Np], r[Np], s[Np]
+  double *trc;             // [K][tr.s][5] states at the trace nodes
+  LimTrace tr;
+  const double *tab;       // ave[Np], then r[Np] and s[Np] centred on
+                           //          their ave-weighted means
   double gamma, min_dens, min_pres;
   unsigned *err;
   BCTable bc;
@@ -63,17 +83,19 @@ __device__ __forceinline__ void lim_prim(double g, const double *c, double *p) {
   p[4] = (g - 1.) * (c[4] - 0.5 * (c[1] * c[1] + c[2] * c[2] + c[3] * c[3]) / c[0]);
 }
 
-// The state of LE consecutive elements moves between HBM and LDS with
-// coalesced 16-byte accesses by all 64 lanes of a wave; lane e < LE then
-// works on element e from LDS.
+// The state of LB consecutive elements moves between HBM and LDS with
+// coalesced 16-byte accesses by all 64 lanes of a wave.  The average pass
+// takes LE elements per wave, one lane each; the reconstruction LER
+// elements per wave, 64 / LER lanes each.
 constexpr int LE = 32;
+constexpr int LER = 16;
 
-template <int ES>
+template <int ES, int LB>
 __device__ __forceinline__ void lim_stage_in(const double *src, double *lds, int cnt, int lane) {
   const int n = cnt * ES;
-  if (cnt == LE) {
+  if (cnt == LB) {
     // every load of the wave in flight before the first LDS write
-    constexpr int NV2 = LE * ES / 2, NIT = (NV2 + 63) / 64;
+    constexpr int NV2 = LB * ES / 2, NIT = (NV2 + 63) / 64;
     dbl2 buf[NIT];
 #pragma unroll
     for (int j = 0; j < NIT; ++j) {
@@ -90,11 +112,11 @@ __device__ __forceinline__ void lim_stage_in(const double *src, double *lds, int
   }
 }
 
-template <int ES>
+template <int ES, int LB>
 __device__ __forceinline__ void lim_stage_out(const double *lds, double *dst, int cnt, int lane) {
   const int n = cnt * ES;
-  if (cnt == LE) {
-    constexpr int NV2 = LE * ES / 2;
+  if (cnt == LB) {
+    constexpr int NV2 = LB * ES / 2;
 #pragma unroll 4
     for (int i = lane; i < NV2; i += 64) ((dbl2 *)dst)[i] = ((const dbl2 *)lds)[i];
   } else {
@@ -116,7 +138,7 @@ __global__ void __launch_bounds__(64) lim_average_kernel(LimArgs a) {
 #pragma unroll
     for (int i = 0; i < 6; ++i) vk[i] = a.vert[e * 6 + i];
   }
-  lim_stage_in<ES>(a.qin + e0 * ES, st, cnt, lane);
+  lim_stage_in<ES, LE>(a.qin + e0 * ES, st, cnt, lane);
   __syncthreads();
   if (lane >= cnt) return;
   const double *ave = a.tab;
@@ -150,15 +172,22 @@ __global__ void __launch_bounds__(64) lim_average_kernel(LimArgs a) {
   o[5] = xc;
   o[6] = yc;
   o[7] = A0;
+  double *tr = a.trc + e * a.tr.s * 5;
+#pragma unroll
+  for (int sl = 0; sl < LIM_MAX_TR; ++sl) {
+    if (sl >= a.tr.s) break;
+    double q5[5];
+    load_euler5<EQN>(q + a.tr.node[sl] * M, q5);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) tr[sl * 5 + c] = q5[c];
+  }
 }
 
 template <int N, int EQN>
 __global__ void __launch_bounds__(256) lim_gradient_kernel(LimArgs a) {
-  constexpr int Np = Order<N>::Np, Nfp = Order<N>::Nfp, M = Vars<EQN>::M;
-  static_assert(N <= 4, "the limiter's face-end reads stay inside 3 Nfp entries for N <= 4");
+  static_assert(N <= 4, "the reference's face-end reads leave its face array for N > 4");
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= a.K) return;
-  const int *F = a.fmask;
   int code[3];
 #pragma unroll
   for (int f = 0; f < 3; ++f) code[f] = a.fcode[(long)e * 3 + f];
@@ -166,7 +195,7 @@ __global__ void __launch_bounds__(256) lim_gradient_kernel(LimArgs a) {
   // src[f] is the element whose averages an interior face f reads, and
   // whose nodes a boundary face f's states read: the last neighbour visited
   // up to face f, or the element itself (:136-233).  Every address below
-  // follows from the face codes and the (uniform) Fmask entries, so all the
+  // follows from the face codes and the (uniform) trace slots, so all the
   // reads of the pass are issued together, before any arithmetic.
   long src[3];
   {
@@ -181,20 +210,7 @@ __global__ void __launch_bounds__(256) lim_gradient_kernel(LimArgs a) {
   // p = element-node id of the end (:241-265).  The ends are the vertex
   // nodes (v1, v2, v2, v3, v3, v1): each face's node list starts where the
   // previous one ends (checked on the host), so three evaluations serve.
-  // End i lies on face fe[i] at position je[i]; the neighbour across it
-  // reads its face node Nfp-1-je[i] of whichever face nf it shares.
-  constexpr int pe[3] = {0, Nfp - 1, 2 * Nfp - 1};   // positions in F of the ends
-  int nodeq[3], fe[3], nbr[3][3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    const int p = F[pe[i]];
-    fe[i] = p / Nfp;
-    const int j = p - fe[i] * Nfp;
-    nodeq[i] = F[p];
-#pragma unroll
-    for (int nf = 0; nf < 3; ++nf) nbr[i][nf] = F[nf * Nfp + Nfp - 1 - j];
-  }
-
+  // Their node states come from the trace slots (LimTrace).
   double av[4][8];   // own row, then the rows of src[0..2]
 #pragma unroll
   for (int w = 0; w < 8; ++w) av[0][w] = a.avg[(long)e * 8 + w];
@@ -202,18 +218,22 @@ __global__ void __launch_bounds__(256) lim_gradient_kernel(LimArgs a) {
   for (int f = 0; f < 3; ++f)
 #pragma unroll
     for (int w = 0; w < 8; ++w) av[f + 1][w] = a.avg[src[f] * 8 + w];
+  const long S = a.tr.s;
   double qm[3][5], qo[3][5];
-  const double *qe = a.qin + (long)e * Np * M;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    const int f = fe[i];
+    const int f = a.tr.face[i];
     const int cf = f == 0 ? code[0] : (f == 1 ? code[1] : code[2]);
     const long sf = f == 0 ? src[0] : (f == 1 ? src[1] : src[2]);
     const int nf = cf & 3;
-    const int nn = nf == 0 ? nbr[i][0] : (nf == 1 ? nbr[i][1] : nbr[i][2]);
-    const long row = cf >= 0 ? (long)(cf >> 2) * Np + nn : sf * Np + nodeq[i];
-    load_euler5<EQN>(qe + nodeq[i] * M, qm[i]);
-    load_euler5<EQN>(a.qin + row * M, qo[i]);
+    const int ns = nf == 0 ? a.tr.nbr[i][0] : (nf == 1 ? a.tr.nbr[i][1] : a.tr.nbr[i][2]);
+    const long row = cf >= 0 ? (long)(cf >> 2) * S + ns : sf * S + a.tr.own[i];
+    const double *pm = a.trc + ((long)e * S + a.tr.own[i]) * 5, *po = a.trc + row * 5;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      qm[i][c] = pm[c];
+      qo[i][c] = po[c];
+    }
   }
   const double *vk = a.vert + (long)e * 6;
   const double xv1 = vk[0], yv1 = vk[1], xv2 = vk[2], yv2 = vk[3], xv3 = vk[4], yv3 = vk[5];
@@ -255,7 +275,7 @@ __global__ void __launch_bounds__(256) lim_gradient_kernel(LimArgs a) {
   double fp[6][5];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    const int f = fe[i];
+    const int f = a.tr.face[i];
     const int cf = f == 0 ? code[0] : (f == 1 ? code[1] : code[2]);
     double qp[5];
     if (cf >= 0) {
@@ -306,19 +326,20 @@ __global__ void __launch_bounds__(256) lim_gradient_kernel(LimArgs a) {
   }
 }
 
-// One element of the reconstruction pass, shared by two lanes: both form
-// the limited gradients, lane `half` reconstructs nodes [half*NH, half*NH+NH)
-// and writes them to out (the element's Np x M block).
+// One element of the reconstruction pass, shared by P = 64 / LER lanes:
+// each forms the limited gradients, lane `part` reconstructs nodes
+// [part*NH, part*NH + NH) and writes them to out (the element's Np x M
+// block).
 template <int N, int EQN>
-__device__ __forceinline__ void lim_reconstruct_element(const LimArgs &a, long e, int half,
+__device__ __forceinline__ void lim_reconstruct_element(const LimArgs &a, long e, int part,
                                                         double *out) {
-  constexpr int Np = Order<N>::Np, M = Vars<EQN>::M, NH = (Np + 1) / 2;
-  const double *ave = a.tab, *rn = a.tab + Np, *sn = a.tab + 2 * Np;
+  constexpr int Np = Order<N>::Np, M = Vars<EQN>::M, P = 64 / LER, NH = (Np + P - 1) / P;
+  const double *cr = a.tab + Np, *cs = a.tab + 2 * Np;
   int code[3];
 #pragma unroll
   for (int f = 0; f < 3; ++f) code[f] = a.fcode[(long)e * 3 + f];
   const double *vk = a.vert + (long)e * 6;
-  const double v0 = vk[0], v1 = vk[1], v2 = vk[2], v3 = vk[3], v4 = vk[4], v5 = vk[5];
+  const double dx1 = vk[2] - vk[0], dy1 = vk[3] - vk[1], dx2 = vk[4] - vk[0], dy2 = vk[5] - vk[1];
   double AC[5];
 #pragma unroll
   for (int c = 0; c < 5; ++c) AC[c] = a.avg[(long)e * 8 + c];
@@ -346,39 +367,34 @@ __device__ __forceinline__ void lim_reconstruct_element(const LimArgs &a, long e
     const double g3 = gx[2][c] * gx[2][c] + gy[2][c] * gy[2][c];
     const double fac = g1 * g1 + g2 * g2 + g3 * g3;
     const double epse = 1.e-14;
-    const double fac3e = fac + 3.0 * epse;
-    const double w1 = (g2 * g3 + epse) / fac3e;
-    const double w2 = (g1 * g3 + epse) / fac3e;
-    const double w3 = (g1 * g2 + epse) / fac3e;
+    // the three weights share one reciprocal of fac + 3 eps (>= 3e-14)
+    const double ifac = 1.0 / (fac + 3.0 * epse);
+    const double w1 = (g2 * g3 + epse) * ifac;
+    const double w2 = (g1 * g3 + epse) * ifac;
+    const double w3 = (g1 * g2 + epse) * ifac;
     Lx[c] = w1 * gx[0][c] + w2 * gx[1][c] + w3 * gx[2][c];
     Ly[c] = w1 * gy[0][c] + w2 * gy[1][c] + w3 * gy[2][c];
   }
 
-  // node coordinates, as the geometry builds them (wxnodaldggeometry2d.cc:211-218),
-  // and their offsets from the element's node average.  The reference
-  // applies dropAVE = I - 1 ave^T as a matrix (:331-340); x_n - sum_j ave_j x_j
-  // is the same product, rounded differently.
-  double xb = 0.0, yb = 0.0;
-#pragma unroll
-  for (int j = 0; j < Np; ++j) {
-    const double r = rn[j], s = sn[j];
-    xb += ave[j] * (0.5 * (-v0 * (r + s) + v2 * (1. + r) + v4 * (1. + s)));
-    yb += ave[j] * (0.5 * (-v1 * (r + s) + v3 * (1. + r) + v5 * (1. + s)));
-  }
-
   double AP[5];
   lim_prim(a.gamma, AC, AP);
-  const double g = a.gamma;
+  const double g = a.gamma, igm1 = 1.0 / (g - 1.);
   const double averho = AP[0], aveu = AP[1], avev = AP[2], avew = AP[3], avep = AP[4];
   unsigned ef = 0;
-  const int nb = half * NH;
+  const int nb = part * NH;
 #pragma unroll
   for (int k = 0; k < NH; ++k) {
     const int n = nb + k;
     if (n >= Np) break;
-    const double r = rn[n], s = sn[n];
-    const double ldx = 0.5 * (-v0 * (r + s) + v2 * (1. + r) + v4 * (1. + s)) - xb;
-    const double ldy = 0.5 * (-v1 * (r + s) + v3 * (1. + r) + v5 * (1. + s)) - yb;
+    // node n's offset from the element's node average.  The reference
+    // forms the node coordinates (wxnodaldggeometry2d.cc:211-218) and
+    // applies dropAVE = I - 1 ave^T (:331-340); with x_n = x_1 +
+    // ((x_2 - x_1)(1 + r_n) + (x_3 - x_1)(1 + s_n)) / 2 that product is
+    // ((x_2 - x_1) cr_n + (x_3 - x_1) cs_n) / 2 with the centred reference
+    // coordinates cr_n = r_n - sum_j ave_j r_j (and cs_n likewise), free of
+    // the cancellation of the absolute coordinates.
+    const double ldx = 0.5 * (dx1 * cr[n] + dx2 * cs[n]);
+    const double ldy = 0.5 * (dy1 * cr[n] + dy2 * cs[n]);
     double drho = ldx * Lx[0] + ldy * Ly[0];
     const double du = ldx * Lx[1] + ldy * Ly[1];
     const double dv = ldx * Lx[2] + ldy * Ly[2];
@@ -399,14 +415,14 @@ __device__ __forceinline__ void lim_reconstruct_element(const LimArgs &a, long e
     const double Lrhov = AC[2] + averho * dv + drho * avev;
     const double Lrhow = AC[3] + averho * dw + drho * avew;
     const double ke = 0.5 * (Lrhou * Lrhou + Lrhov * Lrhov + Lrhow * Lrhow);
-    double dEner = dp / (g - 1.) + 0.5 * drho * (aveu * aveu + avev * avev + avew * avew) +
+    double dEner = dp * igm1 + 0.5 * drho * (aveu * aveu + avev * avev + avew * avew) +
                    averho * (aveu * du + avev * dv + avew * dw);
     double LEner = AC[4] + dEner;
     double Lp = (g - 1.) * (LEner - ke / Lrho);
     if (Lp < a.min_pres) {
       for (int it = 0; it < 4; ++it) {
         dp *= 0.5;
-        dEner = dp / (g - 1.) + 0.5 * drho * (aveu * aveu + avev * avev + avew * avew) +
+        dEner = dp * igm1 + 0.5 * drho * (aveu * aveu + avev * avev + avew * avew) +
                 averho * (aveu * du + avev * dv + avew * dw);
         LEner = AC[4] + dEner;
         Lp = (g - 1.) * (LEner - ke / Lrho);
@@ -423,18 +439,18 @@ __device__ __forceinline__ void lim_reconstruct_element(const LimArgs &a, long e
   if (ef) atomicOr(a.err, ef);
 }
 
-// LE elements per 64-lane wave, lanes l and l + LE on element l.
+// LER elements per 64-lane wave; lanes l, l + LER, l + 2 LER, ... on element l.
 template <int N, int EQN>
 __global__ void __launch_bounds__(64) lim_reconstruct_kernel(LimArgs a) {
   constexpr int Np = Order<N>::Np, M = Vars<EQN>::M, ES = Np * M;
-  static_assert(2 * LE == 64, "two lanes per element");
-  __shared__ __align__(16) double st[LE * ES];
-  const int lane = threadIdx.x, el = lane % LE, half = lane / LE;
-  const long e0 = (long)blockIdx.x * LE;
-  const int cnt = (int)((a.K - e0) < LE ? (a.K - e0) : LE);
-  if (el < cnt) lim_reconstruct_element<N, EQN>(a, e0 + el, half, st + el * ES);
+  static_assert(64 % LER == 0, "whole lanes per element");
+  __shared__ __align__(16) double st[LER * ES];
+  const int lane = threadIdx.x, el = lane % LER, part = lane / LER;
+  const long e0 = (long)blockIdx.x * LER;
+  const int cnt = (int)((a.K - e0) < LER ? (a.K - e0) : LER);
+  if (el < cnt) lim_reconstruct_element<N, EQN>(a, e0 + el, part, st + el * ES);
   __syncthreads();
-  lim_stage_out<ES>(st, a.qout + e0 * ES, cnt, lane);
+  lim_stage_out<ES, LER>(st, a.qout + e0 * ES, cnt, lane);
 }
 
 // Launch entry of one order (defined in apl_kernels_N<N>.hip for N <= 4).
