@@ -140,46 +140,53 @@ __global__ void __launch_bounds__(64) lim_average_kernel(LimArgs a) {
   }
   lim_stage_in<ES, LE>(a.qin + e0 * ES, st, cnt, lane);
   __syncthreads();
-  if (lane >= cnt) return;
-  const double *ave = a.tab;
-  const double *q = st + lane * ES;
-  double c5[5];
+  if (lane < cnt) {
+    const double *ave = a.tab;
+    const double *q = st + lane * ES;
+    double c5[5];
 #pragma unroll
-  for (int c = 0; c < 5; ++c) c5[c] = 0.0;
+    for (int c = 0; c < 5; ++c) c5[c] = 0.0;
 #pragma unroll
-  for (int n = 0; n < Np; ++n) {
-    double q5[5];
-    load_euler5<EQN>(q + n * M, q5);
+    for (int n = 0; n < Np; ++n) {
+      double q5[5];
+      load_euler5<EQN>(q + n * M, q5);
 #pragma unroll
-    for (int c = 0; c < 5; ++c) c5[c] += ave[n] * q5[c];
+      for (int c = 0; c < 5; ++c) c5[c] += ave[n] * q5[c];
+    }
+    // the element's patch centre and area share (:121-126), which its
+    // neighbours' patches reuse (:136-144): same arithmetic either way
+    // J from the vertices as the geometry computes it (wxnodaldggeometry2d.cc:412-426)
+    const double xr = (vk[2] - vk[0]) / 2, xs = (vk[4] - vk[0]) / 2;
+    const double yr = (vk[3] - vk[1]) / 2, ys = (vk[5] - vk[1]) / 2;
+    const double J = -xs * yr + xr * ys;
+    double xc = 0.0, yc = 0.0, A0 = 0.0;
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk) {
+      xc += ave[kk] * vk[2 * kk];
+      yc += ave[kk] * vk[2 * kk + 1];
+      A0 += 2. / 3. * ave[kk] * J;
+    }
+    double *o = a.avg + e * 8;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) o[c] = c5[c];
+    o[5] = xc;
+    o[6] = yc;
+    o[7] = A0;
   }
-  // the element's patch centre and area share (:121-126), which its
-  // neighbours' patches reuse (:136-144): same arithmetic either way
-  // J from the vertices as the geometry computes it (wxnodaldggeometry2d.cc:412-426)
-  const double xr = (vk[2] - vk[0]) / 2, xs = (vk[4] - vk[0]) / 2;
-  const double yr = (vk[3] - vk[1]) / 2, ys = (vk[5] - vk[1]) / 2;
-  const double J = -xs * yr + xr * ys;
-  double xc = 0.0, yc = 0.0, A0 = 0.0;
+  // the trace of the wave's elements, one contiguous [cnt][S][5] block,
+  // gathered from the LDS copy with consecutive lanes on consecutive words
+  const int S5 = a.tr.s * 5;
+  double *tb = a.trc + e0 * S5;
+  for (int t = lane; t < cnt * S5; t += 64) {
+    const int l = t / S5, rem = t - l * S5, sl = rem / 5, c = rem - sl * 5;
+    int node = a.tr.node[0];
 #pragma unroll
-  for (int kk = 0; kk < 3; ++kk) {
-    xc += ave[kk] * vk[2 * kk];
-    yc += ave[kk] * vk[2 * kk + 1];
-    A0 += 2. / 3. * ave[kk] * J;
-  }
-  double *o = a.avg + e * 8;
-#pragma unroll
-  for (int c = 0; c < 5; ++c) o[c] = c5[c];
-  o[5] = xc;
-  o[6] = yc;
-  o[7] = A0;
-  double *tr = a.trc + e * a.tr.s * 5;
-#pragma unroll
-  for (int sl = 0; sl < LIM_MAX_TR; ++sl) {
-    if (sl >= a.tr.s) break;
-    double q5[5];
-    load_euler5<EQN>(q + a.tr.node[sl] * M, q5);
-#pragma unroll
-    for (int c = 0; c < 5; ++c) tr[sl * 5 + c] = q5[c];
+    for (int k = 1; k < LIM_MAX_TR; ++k) node = sl == k ? a.tr.node[k] : node;
+    const double *q = st + l * ES + node * M;
+    double v;
+    if constexpr (M == 5) v = q[c];
+    else v = c == 3 ? 0.0 : q[c == 4 ? 3 : c];   // EULER4: rho w == 0, E last
+    tb[t] = v;
   }
 }
 
