@@ -5,14 +5,18 @@ Follows what Apollo's mesh path hands to the RHS (src/solvers/apsolver.cc:
 order, vertices are the $Nodes in file order, each cell's cone is its node
 list as written, and boundary faces carry the physical tag of the line
 element (type 1) on the same vertex pair as their "Face Sets" label.
+A path ending in ".gz" is read through gzip.
 """
+import gzip
+
 import numpy as np
 
 from .engine import MeshData
 
 
 def read_gmsh(path):
-    with open(path) as fh:
+    opener = gzip.open if str(path).endswith(".gz") else open
+    with opener(path, "rt") as fh:
         lines = [ln.strip() for ln in fh]
     i = lines.index("$MeshFormat")
     version = lines[i + 1].split()[0]

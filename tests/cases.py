@@ -173,6 +173,76 @@ def ffs_state(case, kind="uniform", seed=0):
     return np.ascontiguousarray(q)
 
 
+# Apollo's other Euler shock decks (examples/unstructuredDG/euler/*/*.pin),
+# each on the mesh it names except the backward-facing step, whose grid2.msh
+# (63280 triangles) is replaced by the deck's own grid.msh (same geometry
+# and boundary tags, 15820 triangles).  Boundary list entry i is physical
+# tag i + 1.  All run at P_ORDER = 1 with cfl = 1/(2(N+1)-1).
+#   name: (mesh fixture, triangles, flux, bcs, limiter (gamma, min_dens,
+#          min_pres) or None, initial state)
+def _uniform_flow(mach, p):
+    def q(x, y, g):
+        rho = np.full_like(x, g)
+        rhou = rho * mach * np.sqrt(g * p / rho)
+        return rho, rhou, np.zeros_like(x), np.full_like(x, p)
+    return q
+
+
+def _bfs_state(x, y, g):
+    # backwardFacingStep.pin: rho = gm, rhou = if(x > 0.5, 0, rho M a)
+    rho, rhou, rhov, p = _uniform_flow(5.09, 5.0)(x, y, g)
+    return rho, np.where(x > 0.5, 0.0, rhou), rhov, p
+
+
+def _explosion_state(x, y, g):
+    # explosion.pin: r = |(x, y) - (1, 1)|, rho = if(r > 0.4, 0.125, 1),
+    # p = if(r > 0.4, 0.1, 1), at rest
+    r = np.sqrt((x - 1.0) * (x - 1.0) + (y - 1.0) * (y - 1.0))
+    z = np.zeros_like(x)
+    return np.where(r > 0.4, 0.125, 1.0), z, z, np.where(r > 0.4, 0.1, 1.0)
+
+
+def _pulse_state(x, y, g):
+    # eulerCircularPulse.pin: rho = rho0 (10 + exp(-10 r^2)), p = p0 (10 +
+    # exp(-10 r^2)), rho0 = gamma, p0 = 1, at rest
+    b = 10.0 + 1.0 * np.exp(-10.0 * (x * x + y * y))
+    z = np.zeros_like(x)
+    return g * b, z, z, 1.0 * b
+
+
+_INFLOW = lambda mach, p: (E.BC_EULER_INFLOW, [mach, 0.0, 1.4, p, 1.4])
+_WALL, _COPY = (E.BC_EULER_WALL, []), (E.BC_COPY, [])
+DECKS = {
+    "bfs": ("bfs_grid.msh.gz", 15820, E.FLUX_HLL, [_INFLOW(5.09, 5.0), _WALL, _COPY],
+            (1.4, 1e-2, 1e-2), _bfs_state),
+    "explosion": ("explosion_gridBox.msh.gz", 16790, E.FLUX_LF,
+                  [(E.BC_EULER_DIRICHLET, [0.125, 0.0, 0.0, 0.0, 1.4, 0.1])],
+                  (1.4, 1e-2, 1e-2), _explosion_state),
+    "pulse": ("pulse_grid2.msh.gz", 3472, E.FLUX_HLL, [_WALL], None, _pulse_state),
+    "scramjet": ("scramjet_grid.msh.gz", 10414, E.FLUX_HLL,
+                 [_WALL, _WALL, _INFLOW(3.0, 5.0), _COPY], (1.4, 1e-2, 1e-2),
+                 _uniform_flow(3.0, 5.0)),
+}
+
+
+def deck_case(name, N=1):
+    from apollo_amd.gmsh import read_gmsh
+    fixture, _, flux, bcs, lim, _ = DECKS[name]
+    mesh = read_gmsh(os.path.join(GOLDEN, "meshes", fixture))
+    return Case(mesh, E.Params(N, E.EQN_EULER5, flux=flux, gas_gamma=1.4, bcs=bcs,
+                               limiter=lim), name)
+
+
+def deck_state(name, case):
+    """The deck's initial state at the nodes, [K][Np][5]."""
+    g = 1.4
+    rho, rhou, rhov, p = DECKS[name][5](case.omesh.x, case.omesh.y, g)
+    q = np.zeros(case.omesh.x.shape + (5,))
+    q[..., 0], q[..., 1], q[..., 2] = rho, rhou, rhov
+    q[..., 4] = p / (g - 1) + 0.5 * (rhou * rhou + rhov * rhov) / rho
+    return np.ascontiguousarray(q)
+
+
 def maxwell_case(n=12, N=5, m=3, bc=E.BC_MAXWELL_TM, flux=E.FLUX_LF, precision=E.PREC_FP64,
                  seed=1, chi=0.0, gph=0.0):
     eqn = E.EQN_MAXWELL_TM3 if m == 3 else E.EQN_PHMAXWELL8

@@ -8,10 +8,13 @@ Reads reference data files as text (nothing is compiled, imported or run):
   differentiation p_Dr/p_Ds, the strong-form lift p_LIFT and NUDG's face
   masks p_Fmask.  Stored as ``nudg_N<N>.npz`` (plain arrays, no pickle).
 * five Gmsh v2.2 meshes from ``examples/unstructuredDG`` copied verbatim into
-  ``meshes/`` (data the reference ships with its example decks).
+  ``meshes/`` (data the reference ships with its example decks), and four
+  larger ones, those of the Euler shock decks, gzip-compressed (``.msh.gz``,
+  the text unchanged).
 
 Usage: python tests/golden/make_fixtures.py [/root/reference]
 """
+import gzip
 import os
 import re
 import shutil
@@ -29,6 +32,13 @@ MESHES = {
     "vortex_grid.msh": "examples/unstructuredDG/euler/isentropicVortex/grid.msh",
     "tm_square.msh": "examples/unstructuredDG/maxwell/transverseMagnetic/square.msh",
     "ffs_grid.msh": "examples/unstructuredDG/euler/forwardFacingStep/grid.msh",
+}
+
+GZ_MESHES = {
+    "pulse_grid2.msh.gz": "examples/unstructuredDG/euler/eulerPulse/grid2.msh",
+    "scramjet_grid.msh.gz": "examples/unstructuredDG/euler/scramjet/grid.msh",
+    "explosion_gridBox.msh.gz": "examples/unstructuredDG/euler/explosionTest/gridBox.msh",
+    "bfs_grid.msh.gz": "examples/unstructuredDG/euler/backwardFacingStep/grid.msh",
 }
 
 
@@ -57,6 +67,12 @@ def main():
     os.makedirs(os.path.join(HERE, "meshes"), exist_ok=True)
     for dst, src in MESHES.items():
         shutil.copyfile(os.path.join(REF, src), os.path.join(HERE, "meshes", dst))
+    for dst, src in GZ_MESHES.items():
+        with open(os.path.join(REF, src), "rb") as fi:
+            data = fi.read()
+        with open(os.path.join(HERE, "meshes", dst), "wb") as fo:
+            with gzip.GzipFile(filename="", mode="wb", fileobj=fo, mtime=0) as gz:
+                gz.write(data)
     print("fixtures written to", HERE)
 
 

@@ -117,7 +117,7 @@ def test_ssp_ffs_deck_matches_oracle(N, nsteps):
     assert len(dts) == len(dto) == nsteps
     assert np.abs(dts - dto).max() <= 1e-12 * dto.max()
     assert err <= max(1e-11, 10.0 * sens)
-    assert rel_l2(q - q0, np.zeros_like(q)) > 1e-6          # the step did act
+    assert rel_l2(q, q0) > 1e-3                              # the steps did act
 
 
 def test_lserk4_with_limiter_matches_oracle():

@@ -99,6 +99,37 @@ def test_host_connectivity_gmsh_fixtures(name):
                           O.facepair(mesh.EToV, mesh.Nv, None, mesh.face_tag))
 
 
+@pytest.mark.parametrize("name", ["bfs", "explosion", "pulse", "scramjet"])
+def test_shock_deck_meshes(name):
+    """The meshes of Apollo's Euler shock decks: the triangle count, every
+    triangle counter-clockwise, every boundary edge (an edge of one triangle
+    only) tagged with an entry of the deck's boundary list and every entry
+    used, and the engine's face pairing equal to the oracle's."""
+    from apollo_amd.gmsh import read_gmsh
+    from cases import DECKS
+    fixture, ntri, _, bcs, _, _ = DECKS[name]
+    m = read_gmsh(os.path.join(GOLDEN, "meshes", fixture))
+    assert m.K == ntri
+    x, y = m.VX[m.EToV], m.VY[m.EToV]
+    area = (x[:, 1] - x[:, 0]) * (y[:, 2] - y[:, 0]) - (x[:, 2] - x[:, 0]) * (y[:, 1] - y[:, 0])
+    assert (area > 0).all()
+    count = {}
+    for k in range(m.K):
+        for f in range(3):
+            a, b = m.EToV[k, f], m.EToV[k, (f + 1) % 3]
+            key = (min(a, b), max(a, b))
+            count[key] = count.get(key, 0) + 1
+    tags = set()
+    for k in range(m.K):
+        for f in range(3):
+            a, b = m.EToV[k, f], m.EToV[k, (f + 1) % 3]
+            if count[(min(a, b), max(a, b))] == 1:
+                assert 1 <= m.face_tag[k, f] <= len(bcs), (k, f, m.face_tag[k, f])
+                tags.add(int(m.face_tag[k, f]))
+    assert tags == set(range(1, len(bcs) + 1))
+    assert np.array_equal(E.host_connectivity(m), O.facepair(m.EToV, m.Nv, None, m.face_tag))
+
+
 def test_gmsh_reader():
     from apollo_amd.gmsh import read_gmsh
     mesh = read_gmsh(os.path.join(GOLDEN, "meshes", "advection_test.msh"))
