@@ -70,6 +70,7 @@ struct apl_handle_s {
   // states at the trace nodes
   double *d_lavg = nullptr, *d_lgrad = nullptr, *d_ltab = nullptr, *d_ltrc = nullptr;
   LimTrace ltr{};
+  bool ltr_is_state = false;   // trace slots = nodes 0..Np-1 of a 5-variable state
 };
 
 // ---------------------------------------------------------------------------
@@ -656,7 +657,10 @@ int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out)
     if ((rc = dev_upload<double>(h, &h->d_ltab, tab.data(), tab.size()))) return bail(rc);
     if ((rc = dev_alloc(h, (void **)&h->d_lavg, sizeof(double) * 8 * (size_t)h->K))) return bail(rc);
     if ((rc = dev_alloc(h, (void **)&h->d_lgrad, sizeof(double) * 40 * (size_t)h->K))) return bail(rc);
-    if ((rc = dev_alloc(h, (void **)&h->d_ltrc, sizeof(double) * 5 * (size_t)tr.s * h->K)))
+    h->ltr_is_state = tr.s == Np && params->eqn == APL_EQN_EULER5;
+    for (int q = 0; q < tr.s && h->ltr_is_state; ++q) h->ltr_is_state = tr.node[q] == q;
+    if (!h->ltr_is_state &&
+        (rc = dev_alloc(h, (void **)&h->d_ltrc, sizeof(double) * 5 * (size_t)tr.s * h->K)))
       return bail(rc);
   }
   *out = h;
@@ -675,6 +679,8 @@ static int enqueue_limit(apl_handle *h, const double *qin, double *qout) {
   a.vert = h->d_vert;
   a.fcode = h->d_fcode;
   a.trc = h->d_ltrc;
+  a.trc_rd = h->ltr_is_state ? qin : h->d_ltrc;
+  a.trc_write = h->ltr_is_state ? 0 : 1;
   a.tr = h->ltr;
   a.tab = h->d_ltab;
   a.gamma = h->params.lim_gas_gamma;

@@ -65,7 +65,12 @@ struct LimArgs {
   const double *geom;      // [K][G_WORDS]
   const double *vert;      // [K][6]
   const int *fcode;        // [K][3]
-  double *trc;             // [K][tr.s][5] states at the trace nodes
+  double *trc;             // [K][tr.s][5] states at the trace nodes, written by
+                           //          the average pass when trc_write
+  const double *trc_rd;    // the trace the gradient pass reads: trc, or the
+                           //          state itself when the slots are nodes
+                           //          0 .. Np-1 of a 5-variable state (N = 1)
+  int trc_write;
   LimTrace tr;
   const double *tab;       // ave[Np], then r[Np] and s[Np] centred on
                            //          their ave-weighted means
@@ -175,6 +180,7 @@ __global__ void __launch_bounds__(64) lim_average_kernel(LimArgs a) {
   }
   // the trace of the wave's elements, one contiguous [cnt][S][5] block,
   // gathered from the LDS copy with consecutive lanes on consecutive words
+  if (!a.trc_write) return;
   const int S5 = a.tr.s * 5;
   double *tb = a.trc + e0 * S5;
   for (int t = lane; t < cnt * S5; t += 64) {
@@ -235,7 +241,7 @@ __global__ void __launch_bounds__(256) lim_gradient_kernel(LimArgs a) {
     const int nf = cf & 3;
     const int ns = nf == 0 ? a.tr.nbr[i][0] : (nf == 1 ? a.tr.nbr[i][1] : a.tr.nbr[i][2]);
     const long row = cf >= 0 ? (long)(cf >> 2) * S + ns : sf * S + a.tr.own[i];
-    const double *pm = a.trc + ((long)e * S + a.tr.own[i]) * 5, *po = a.trc + row * 5;
+    const double *pm = a.trc_rd + ((long)e * S + a.tr.own[i]) * 5, *po = a.trc_rd + row * 5;
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
       qm[i][c] = pm[c];

@@ -81,14 +81,15 @@ def workload(name):
                 "seed 3, K=131072, N=3, LF flux, J = int_{r<1} rho^2 at step 200, checkpoint 20")
         return dict(name=name, mesh=mesh, params=params, cfl=0.5, desc=desc, m=4, analytic="vortex",
                     L=10.0)
-    if name == "c6":
-        n, N = 512, 4
+    if name in ("c6", "c6n1"):
+        # c6n1: the same at N = 1, the order of Apollo's shock decks
+        n, N = 512, (4 if name == "c6" else 1)
         mesh = E.structured_mesh(n, -10.0, 10.0, -10.0, 10.0, 0.15, 5, False)
         params = dict(order=N, eqn=E.EQN_EULER5, flux=E.FLUX_HLL, bcs=[(E.BC_EULER_WALL, [])],
                       limiter=(1.4, 1e-2, 1e-2))
         desc = ("Hesthaven-Warburton Euler limiter (apl_limit, in place): Apollo's 5-variable "
-                "Euler, [-10,10]^2 with walls, 512x512x2 jittered 0.15h seed 5, K=524288, N=4, "
-                "isentropic-vortex state")
+                "Euler, [-10,10]^2 with walls, 512x512x2 jittered 0.15h seed 5, K=524288, N=%d, "
+                "isentropic-vortex state" % N)
         return dict(name=name, mesh=mesh, params=params, cfl=0.5, desc=desc, m=5, analytic="vortex5",
                     L=20.0)
     raise SystemExit("unknown workload %s" % name)
@@ -306,12 +307,12 @@ def bytes_alg_limit(K, Np, m):
     return K * (2 * Np * m * 8 + 140)
 
 
-def run_limiter(torch, reps=50):
-    """Time apl_limit in place on the c6 workload with HIP events on the
-    engine's stream; report applications/s, DOF/s and the algorithmic-byte
-    rate against the HBM peak."""
+def run_limiter(torch, reps=50, name="c6"):
+    """Time apl_limit in place on a limiter workload (c6: N = 4, c6n1:
+    N = 1) with HIP events on the engine's stream; report applications/s,
+    DOF/s and the algorithmic-byte rate against the HBM peak."""
     from apollo_amd import engine as E
-    w = workload("c6")
+    w = workload(name)
     eng, stream = engine_on_stream(w, E.PREC_FP64, E.ALGO_AUTO, torch)
     g, x, y = eng.geometry()
     q0 = np.ascontiguousarray(E.vortex_state(x, y, 0.0, m=5))
@@ -442,7 +443,8 @@ def main():
     from apollo_amd import engine as E
     assert torch.cuda.is_available()
     if args.workload == "c6":
-        print(json.dumps({"limiter_c6": run_limiter(torch)}), flush=True)
+        print(json.dumps({"limiter_c6": run_limiter(torch),
+                          "limiter_c6_n1": run_limiter(torch, name="c6n1")}), flush=True)
         return
     w = workload(args.workload)
     linear_ok = w["m"] == 3
@@ -549,6 +551,7 @@ def main():
         line["maxwell_c5"] = run_leg("c5", 300, three, torch)
         print("bench: limiter (c6)", file=sys.stderr, flush=True)
         line["limiter_c6"] = run_limiter(torch)
+        line["limiter_c6_n1"] = run_limiter(torch, name="c6n1")
     if not args.no_cpu_baseline:
         nth = min(16, os.cpu_count() or 1)
         print("bench: cpu baseline, %d threads" % nth, file=sys.stderr, flush=True)
