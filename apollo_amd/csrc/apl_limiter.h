@@ -7,7 +7,9 @@
 //                           :487-522), one FP64 5-vector per element, and
 //                           the states at the few nodes the gradient pass
 //                           reads (LimTrace), so that it gathers from a
-//                           compact array instead of the full state;
+//                           compact array instead of the full state (at
+//                           N = 1 those nodes are the whole state, which it
+//                           then reads directly);
 //   lim_gradient_kernel     the four-element patch of each element: patch
 //                           centres and areas, ghost averages through the
 //                           boundary states, face averages at the six face
@@ -18,12 +20,13 @@
 //                           and the reconstruction with its density and
 //                           pressure floors (:317-467).
 //
-// The average and reconstruction passes move the state of 32 consecutive
-// elements per wave between HBM and LDS with coalesced 16-byte accesses;
-// the average pass works one lane per element on the LDS copy, the
-// reconstruction two lanes per element (half of its nodes each).  The
-// gradient pass, a gather of a few hundred bytes per element, runs one
-// thread per element and issues all of its reads at once.  The
+// The average and reconstruction passes move the state of consecutive
+// elements between HBM and LDS with coalesced 16-byte accesses: the average
+// pass 32 elements per wave, one lane per element on the LDS copy; the
+// reconstruction 16 elements per wave, four lanes per element (a quarter
+// of its nodes each).  The gradient pass, a gather of a few hundred bytes
+// per element, runs one thread per element and issues all of its reads at
+// once.  The
 // reference's reading order and quirks are kept (see the oracle's
 // orc_limit): AVE[0..2] with the vertex nodes for the patch centres, the
 // boundary-face node states read from the last neighbour visited, the face
