@@ -71,6 +71,9 @@ struct apl_handle_s {
   double *d_lavg = nullptr, *d_lgrad = nullptr, *d_ltab = nullptr, *d_ltrc = nullptr;
   LimTrace ltr{};
   bool ltr_is_state = false;   // trace slots = nodes 0..Np-1 of a 5-variable state
+  // pinned host words the flag and speed read-backs land in: [0] flags
+  // (unsigned), [1] the step's speed maximum (double)
+  double *h_pin = nullptr;
 };
 
 // ---------------------------------------------------------------------------
@@ -162,10 +165,13 @@ static RhsArgs base_args(apl_handle *h) {
   return a;
 }
 
+// Read the error flags after everything enqueued so far (one stream
+// synchronisation).
 static int check_flags(apl_handle *h) {
-  unsigned ef = 0;
-  HIPCHK(h, hipMemcpyAsync(&ef, h->d_err, sizeof(unsigned), hipMemcpyDeviceToHost, h->stream));
+  unsigned *pf = (unsigned *)h->h_pin;
+  HIPCHK(h, hipMemcpyAsync(pf, h->d_err, sizeof(unsigned), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  const unsigned ef = *pf;
   if (ef & EF_DENSITY) return fail(h, APL_ERR_NEG_DENSITY, "negative density in the Euler flux");
   if (ef & EF_PRESSURE) return fail(h, APL_ERR_NEG_PRESSURE, "negative pressure in the Euler flux");
   if (ef & EF_NAN) return fail(h, APL_ERR_NAN, "NaN in the RHS");
@@ -396,6 +402,7 @@ static void free_handle(apl_handle *h) {
                   h->d_seg, h->d_opart, h->d_lavg, h->d_lgrad, h->d_ltab, h->d_ltrc};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
+  if (h->h_pin) (void)hipHostFree(h->h_pin);
   for (double *b : h->d_adjops)
     if (b) (void)hipFree(b);
   if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
@@ -600,6 +607,11 @@ int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out)
   else rc = upload_fragments_bf16x3(h);
   if (rc) return bail(rc);
   if ((rc = dev_alloc(h, (void **)&h->d_err, sizeof(unsigned) * 4))) return bail(rc);
+  if (hipHostMalloc((void **)&h->h_pin, 2 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    h->h_pin = nullptr;
+    h->last_error = "pinned host allocation failed";
+    return bail(APL_ERR_HIP);
+  }
   if ((rc = dev_alloc(h, (void **)&h->d_speed, sizeof(double) * ((h->K + 15) / 16)))) return bail(rc);
   if ((rc = dev_alloc(h, (void **)&h->d_t, sizeof(double) * 2))) return bail(rc);
   if (params->limiter) {
@@ -714,7 +726,6 @@ int apl_limit(apl_handle *h, const double *q_in, double *q_out, int where) {
   } else {
     if ((rc = enqueue_limit(h, q_in, q_out))) return rc;
   }
-  HIPCHK(h, hipStreamSynchronize(h->stream));
   return check_flags(h);
 }
 
@@ -964,7 +975,6 @@ static int run_lserk4(apl_handle *h, double t0, double dt, int nsteps, double *q
   for (int s = 0; s < nsteps; ++s) HIPCHK(h, hipGraphLaunch(h->step_exec[s & 1], h->stream));
   if (nsteps & 1)
     HIPCHK(h, hipMemcpyAsync(qa, qb, sizeof(double) * state_len(h), hipMemcpyDeviceToDevice, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
   return check_flags(h);
 }
 
@@ -1061,12 +1071,13 @@ int apl_ssp_advance(apl_handle *h, double t0, double t_end, double dt0, double c
     a.speed_part = h->d_speed;
     if ((rc = dispatch_rhs(h, a))) return rc;
     if (launch_max_reduce(h->stream, h->d_speed, nb, d_smax)) return fail(h, APL_ERR_HIP, "max kernel");
-    double ms = 0.0;
-    unsigned ef = 0;
-    HIPCHK(h, hipMemcpyAsync(&ms, d_smax, sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipMemcpyAsync(&ef, h->d_err, sizeof(unsigned), hipMemcpyDeviceToHost, h->stream));
+    unsigned *pf = (unsigned *)h->h_pin;
+    double *pm = h->h_pin + 1;
+    HIPCHK(h, hipMemcpyAsync(pm, d_smax, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(pf, h->d_err, sizeof(unsigned), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    if (ef) return check_flags(h);
+    const double ms = *pm;
+    if (*pf) return check_flags(h);
     // wxnodaldg2dmethod.cc:523, then apsolver.cc:474
     const double suggested = 2. / 3. * cfl * h->geo.dtscale * (h->ops.rmin / ms);
     dt_temp = std::fmin(suggested, dt);
