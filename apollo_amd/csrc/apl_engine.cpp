@@ -71,8 +71,7 @@ struct apl_handle_s {
   double *d_lavg = nullptr, *d_lgrad = nullptr, *d_ltab = nullptr, *d_ltrc = nullptr;
   LimTrace ltr{};
   bool ltr_is_state = false;   // trace slots = nodes 0..Np-1 of a 5-variable state
-  // pinned host words the flag and speed read-backs land in: [0] flags
-  // (unsigned), [1] the step's speed maximum (double)
+  // pinned host words the flag and speed read-backs land in (two doubles)
   double *h_pin = nullptr;
 };
 
@@ -397,7 +396,7 @@ static void free_handle(apl_handle *h) {
   if (!h) return;
   free_graphs(h);
   void *bufs[] = {h->d_geom, h->d_vert, h->d_I1D, h->d_fcode, h->d_fmask, h->d_A1, h->d_A2,
-                  h->d_A3, h->d_err, h->d_speed, h->d_t, h->d_qa, h->d_qb, h->d_res, h->d_out,
+                  h->d_A3, h->d_speed, h->d_t, h->d_qa, h->d_qb, h->d_res, h->d_out,
                   h->d_ckpt, h->d_Vc, h->d_cub, h->d_sbar, h->d_qbar, h->d_rbar, h->d_stg,
                   h->d_seg, h->d_opart, h->d_lavg, h->d_lgrad, h->d_ltab, h->d_ltrc};
   for (void *b : bufs)
@@ -606,14 +605,16 @@ int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out)
   else if (params->precision == APL_PREC_FP32_MFMA) rc = upload_fragments<float>(h);
   else rc = upload_fragments_bf16x3(h);
   if (rc) return bail(rc);
-  if ((rc = dev_alloc(h, (void **)&h->d_err, sizeof(unsigned) * 4))) return bail(rc);
   if (hipHostMalloc((void **)&h->h_pin, 2 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
     h->h_pin = nullptr;
     h->last_error = "pinned host allocation failed";
     return bail(APL_ERR_HIP);
   }
   if ((rc = dev_alloc(h, (void **)&h->d_speed, sizeof(double) * ((h->K + 15) / 16)))) return bail(rc);
-  if ((rc = dev_alloc(h, (void **)&h->d_t, sizeof(double) * 2))) return bail(rc);
+  // d_t: [0] device clock, [1] a step's speed maximum, [2] the error flags
+  // (an unsigned), so that one copy reads back both of a step's results
+  if ((rc = dev_alloc(h, (void **)&h->d_t, sizeof(double) * 4))) return bail(rc);
+  h->d_err = (unsigned *)(h->d_t + 2);
   if (params->limiter) {
     // the limiter's face ends are the vertex nodes shared by consecutive
     // faces: Fmask[f][Nfp-1] == Fmask[f+1][0]
@@ -1071,13 +1072,11 @@ int apl_ssp_advance(apl_handle *h, double t0, double t_end, double dt0, double c
     a.speed_part = h->d_speed;
     if ((rc = dispatch_rhs(h, a))) return rc;
     if (launch_max_reduce(h->stream, h->d_speed, nb, d_smax)) return fail(h, APL_ERR_HIP, "max kernel");
-    unsigned *pf = (unsigned *)h->h_pin;
-    double *pm = h->h_pin + 1;
-    HIPCHK(h, hipMemcpyAsync(pm, d_smax, sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipMemcpyAsync(pf, h->d_err, sizeof(unsigned), hipMemcpyDeviceToHost, h->stream));
+    // the speed maximum and the flags word after it, in one copy
+    HIPCHK(h, hipMemcpyAsync(h->h_pin, d_smax, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    const double ms = *pm;
-    if (*pf) return check_flags(h);
+    const double ms = h->h_pin[0];
+    if (*(const unsigned *)(h->h_pin + 1)) return check_flags(h);
     // wxnodaldg2dmethod.cc:523, then apsolver.cc:474
     const double suggested = 2. / 3. * cfl * h->geo.dtscale * (h->ops.rmin / ms);
     dt_temp = std::fmin(suggested, dt);
