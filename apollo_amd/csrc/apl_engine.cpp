@@ -669,6 +669,17 @@ int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out)
     }
     if ((rc = dev_upload<double>(h, &h->d_ltab, tab.data(), tab.size()))) return bail(rc);
     if ((rc = dev_alloc(h, (void **)&h->d_lavg, sizeof(double) * 8 * (size_t)h->K))) return bail(rc);
+    {
+      LimArgs pa{};
+      pa.K = h->K;
+      pa.vert = h->d_vert;
+      pa.tab = h->d_ltab;
+      pa.avg = h->d_lavg;
+      if (launch_limiter_patch(h->stream, pa) != APL_OK || hipStreamSynchronize(h->stream) != hipSuccess) {
+        h->last_error = "limiter patch geometry kernel failed";
+        return bail(APL_ERR_HIP);
+      }
+    }
     if ((rc = dev_alloc(h, (void **)&h->d_lgrad, sizeof(double) * 40 * (size_t)h->K))) return bail(rc);
     h->ltr_is_state = tr.s == Np && params->eqn == APL_EQN_EULER5;
     for (int q = 0; q < tr.s && h->ltr_is_state; ++q) h->ltr_is_state = tr.node[q] == q;

@@ -142,6 +142,32 @@ int launch_limiter(int eqn, hipStream_t s, const LimArgs &a) {
 template int launch_limiter<APL_ORDER>(int, hipStream_t, const LimArgs &);
 
 #if APL_ORDER == 1
+__global__ void __launch_bounds__(256) lim_patch_kernel(LimArgs a) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.K) return;
+  const double *ave = a.tab, *vk = a.vert + e * 6;
+  // J from the vertices as the geometry computes it (wxnodaldggeometry2d.cc:412-426)
+  const double xr = (vk[2] - vk[0]) / 2, xs = (vk[4] - vk[0]) / 2;
+  const double yr = (vk[3] - vk[1]) / 2, ys = (vk[5] - vk[1]) / 2;
+  const double J = -xs * yr + xr * ys;
+  double xc = 0.0, yc = 0.0, A0 = 0.0;
+#pragma unroll
+  for (int kk = 0; kk < 3; ++kk) {
+    xc += ave[kk] * vk[2 * kk];
+    yc += ave[kk] * vk[2 * kk + 1];
+    A0 += 2. / 3. * ave[kk] * J;
+  }
+  double *o = a.avg + e * 8;
+  o[5] = xc;
+  o[6] = yc;
+  o[7] = A0;
+}
+
+int launch_limiter_patch(hipStream_t s, const LimArgs &a) {
+  hipLaunchKernelGGL(lim_patch_kernel, dim3((a.K + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError() == hipSuccess ? APL_OK : APL_ERR_HIP;
+}
+
 __global__ void advance_time_kernel(double *t, double dt) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *t = *t + dt;
 }

@@ -2,6 +2,8 @@
 // (src/hyperapps/euler/wxhesthavenwarburtoneulerlimiter.cc:61-484) on the
 // GPU, as three element-parallel passes over HBM-resident state:
 //
+//   lim_patch_kernel        (once per handle) each element's patch centre
+//                           and area share, mesh constants;
 //   lim_average_kernel      cell averages of the conserved variables
 //                           (computeConservedAndPrimitiveAVEVariables,
 //                           :487-522), one FP64 5-vector per element, and
@@ -58,8 +60,9 @@ struct LimArgs {
   int K;
   const double *qin;       // [K][Np][M]
   double *qout;            // [K][Np][M], may equal qin
-  double *avg;             // [K][8]   conserved cell averages, then the
-                           //          patch centre xc, yc and area share A0
+  double *avg;             // [K][8]   conserved cell averages (every call),
+                           //          then the patch centre xc, yc and
+                           //          area share A0 (once per handle)
   double *grad;            // [K][2][5] cell gradients C0x, C0y, then
                            //          [K][6][5] edge gradients E1x, E1y,
                            //          E2x, E2y, E3x, E3y, written only for
@@ -139,13 +142,7 @@ __global__ void __launch_bounds__(64) lim_average_kernel(LimArgs a) {
   const int lane = threadIdx.x;
   const long e0 = (long)blockIdx.x * LE;
   const int cnt = (int)((a.K - e0) < LE ? (a.K - e0) : LE);
-  // the element's vertices, read together with the state
   const long e = e0 + lane;
-  double vk[6];
-  if (lane < cnt) {
-#pragma unroll
-    for (int i = 0; i < 6; ++i) vk[i] = a.vert[e * 6 + i];
-  }
   lim_stage_in<ES, LE>(a.qin + e0 * ES, st, cnt, lane);
   __syncthreads();
   if (lane < cnt) {
@@ -161,25 +158,10 @@ __global__ void __launch_bounds__(64) lim_average_kernel(LimArgs a) {
 #pragma unroll
       for (int c = 0; c < 5; ++c) c5[c] += ave[n] * q5[c];
     }
-    // the element's patch centre and area share (:121-126), which its
-    // neighbours' patches reuse (:136-144): same arithmetic either way
-    // J from the vertices as the geometry computes it (wxnodaldggeometry2d.cc:412-426)
-    const double xr = (vk[2] - vk[0]) / 2, xs = (vk[4] - vk[0]) / 2;
-    const double yr = (vk[3] - vk[1]) / 2, ys = (vk[5] - vk[1]) / 2;
-    const double J = -xs * yr + xr * ys;
-    double xc = 0.0, yc = 0.0, A0 = 0.0;
-#pragma unroll
-    for (int kk = 0; kk < 3; ++kk) {
-      xc += ave[kk] * vk[2 * kk];
-      yc += ave[kk] * vk[2 * kk + 1];
-      A0 += 2. / 3. * ave[kk] * J;
-    }
+    // words 5..7 of the row hold the patch geometry (lim_patch_kernel)
     double *o = a.avg + e * 8;
 #pragma unroll
     for (int c = 0; c < 5; ++c) o[c] = c5[c];
-    o[5] = xc;
-    o[6] = yc;
-    o[7] = A0;
   }
   // the trace of the wave's elements, one contiguous [cnt][S][5] block,
   // gathered from the LDS copy with consecutive lanes on consecutive words
@@ -476,5 +458,11 @@ __global__ void __launch_bounds__(64) lim_reconstruct_kernel(LimArgs a) {
 // Launch entry of one order (defined in apl_kernels_N<N>.hip for N <= 4).
 template <int N>
 int launch_limiter(int eqn, hipStream_t s, const LimArgs &a);
+
+// The mesh constants of the averages rows, once per handle: each element's
+// patch centre xc, yc and area share A0 (:121-126), which its neighbours'
+// patches reuse (:136-144), into words 5..7 of its row.  Needs a.K, a.vert,
+// a.tab (the weights of the vertex nodes) and a.avg.
+int launch_limiter_patch(hipStream_t s, const LimArgs &a);
 
 }  // namespace apl
