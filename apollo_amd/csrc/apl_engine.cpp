@@ -692,7 +692,9 @@ int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out)
 }
 
 // The limiter's three passes on the handle's stream (qout may equal qin).
-static int enqueue_limit(apl_handle *h, const double *qin, double *qout) {
+// clear_err: the first pass zeroes the error flags (apl_limit's own call;
+// the steppers keep the flags of the step's earlier kernels).
+static int enqueue_limit(apl_handle *h, const double *qin, double *qout, bool clear_err = false) {
   LimArgs a;
   a.K = h->K;
   a.qin = qin;
@@ -705,6 +707,7 @@ static int enqueue_limit(apl_handle *h, const double *qin, double *qout) {
   a.trc = h->d_ltrc;
   a.trc_rd = h->ltr_is_state ? qin : h->d_ltrc;
   a.trc_write = h->ltr_is_state ? 0 : 1;
+  a.clear_err = clear_err ? 1 : 0;
   a.tr = h->ltr;
   a.tab = h->d_ltab;
   a.gamma = h->params.lim_gas_gamma;
@@ -729,14 +732,13 @@ int apl_limit(apl_handle *h, const double *q_in, double *q_out, int where) {
   if (!h->params.limiter) return fail(h, APL_ERR_ARG, "apl_limit: handle created without the limiter");
   int rc;
   if ((rc = ensure_work(h))) return rc;
-  HIPCHK(h, hipMemsetAsync(h->d_err, 0, sizeof(unsigned), h->stream));
   if (where == APL_HOST) {
     const size_t bytes = sizeof(double) * state_len(h);
     HIPCHK(h, hipMemcpyAsync(h->d_qa, q_in, bytes, hipMemcpyHostToDevice, h->stream));
-    if ((rc = enqueue_limit(h, h->d_qa, h->d_qa))) return rc;
+    if ((rc = enqueue_limit(h, h->d_qa, h->d_qa, true))) return rc;
     HIPCHK(h, hipMemcpyAsync(q_out, h->d_qa, bytes, hipMemcpyDeviceToHost, h->stream));
   } else {
-    if ((rc = enqueue_limit(h, q_in, q_out))) return rc;
+    if ((rc = enqueue_limit(h, q_in, q_out, true))) return rc;
   }
   return check_flags(h);
 }

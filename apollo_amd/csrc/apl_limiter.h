@@ -77,6 +77,8 @@ struct LimArgs {
                            //          state itself when the slots are nodes
                            //          0 .. Np-1 of a 5-variable state (N = 1)
   int trc_write;
+  int clear_err;           // the average pass zeroes *err first (the passes
+                           //          before the reconstruction set no flags)
   LimTrace tr;
   const double *tab;       // ave[Np], then r[Np] and s[Np] centred on
                            //          their ave-weighted means
@@ -143,6 +145,7 @@ __global__ void __launch_bounds__(64) lim_average_kernel(LimArgs a) {
   const long e0 = (long)blockIdx.x * LE;
   const int cnt = (int)((a.K - e0) < LE ? (a.K - e0) : LE);
   const long e = e0 + lane;
+  if (a.clear_err && blockIdx.x == 0 && lane == 0) *a.err = 0u;
   lim_stage_in<ES, LE>(a.qin + e0 * ES, st, cnt, lane);
   __syncthreads();
   if (lane < cnt) {

@@ -82,6 +82,27 @@ def test_limit_needs_a_limiter_handle():
     eng.close()
 
 
+@pytest.mark.parametrize("N", [1, 4])
+def test_limit_reports_nan_then_starts_clean(N):
+    """A NaN in the state reaches the limited state and apl_limit reports
+    APL_ERR_NAN (7, as wxnodaldg2dmethod.cc:246 does for the RHS); the next
+    call on the same handle starts from cleared flags, succeeds and matches
+    the oracle."""
+    case = ffs_case(N=N)
+    q = ffs_state(case, "smooth", seed=3)
+    bad = q.copy()
+    bad[5, 0, 0] = np.nan
+    eng = E.Engine(case.mesh, case.params)
+    with pytest.raises(E.ApolloError) as ei:
+        eng.limit(bad, np.zeros_like(q))
+    assert ei.value.status == 7
+    out = np.zeros_like(q)
+    eng.limit(q, out)
+    eng.close()
+    want = from_oracle(case.params.eqn, O.limit(case.omesh, case.ophys, to_oracle(case.params.eqn, q)))
+    _close(out, want)
+
+
 @pytest.mark.parametrize("nsteps", [4, 13])
 @pytest.mark.parametrize("N", [1, 2])
 def test_ssp_ffs_deck_matches_oracle(N, nsteps):
