@@ -1,31 +1,34 @@
 #!/usr/bin/env python3
 """Benchmark: DG RHS DOF-updates/s on one MI355X.
 
-Default workload (Maxwell TM cavity, the configuration the headline metric
-is quoted on): 256 x 256 quads split into 2 triangles (K = 131072), vertices
-jittered +-0.1 h (seed 1), N = 5 (Np = 21), PEC walls with Apollo's
+Headline workload (tm_cavity_256_N5, the configuration the metric is quoted
+on): 2D Maxwell TM, 256 x 256 quads split into 2 triangles (K = 131072),
+vertices jittered +-0.1 h (seed 1), N = 5 (Np = 21), PEC walls with Apollo's
 transverse-magnetic boundary state, upwind flux, Ez = sin(3 pi x) sin(4 pi y),
-H = 0, LSERK4 with Apollo's dt formula at the deck CFL 0.5/(2N+1), fixed dt,
-500 steps.
+H = 0, LSERK4 with Apollo's dt formula at the deck CFL 0.5/(2N+1), fixed dt.
 
 A "step" is one LSERK4 step = 5 RHS evaluations fused with the stage
 updates, replayed as one hipGraph.  DOF-updates/s = K Np m x (RHS
-evaluations) / time with m = 3 (Ez, Bx, By).  `value` is the FP32-MFMA
-engine; the FP64 engine and Apollo's over-integrated cubature form are timed
-the same way and reported beside it, each with its L2 error against the
-analytic cavity mode (and FP32 against FP64).  State lives in HBM before the
-timed region starts.
+evaluations) / time with m = 3 (Ez, Bx, By).  `value` is the FP64 engine
+(the reference's precision); FP32-MFMA and split-BF16x3 contractions and
+Apollo's over-integrated cubature form are timed the same way beside it,
+each with its speed-up over FP64 and its relative L2 error against the
+analytic cavity mode and against the FP64 run after 500 steps (a fixed-length
+accuracy run, whatever --steps is).  State lives in HBM before the timed
+region starts.
 
-The adjoint leg (Euler isentropic vortex, [-5,5]^2 periodic, 256x256x2
-jittered 0.1h seed 3, N = 3, 200 steps, checkpoint every 20) times the
-reverse LSERK4 and checks dJ/dq0 against central finite differences along 8
-Gaussian directions (eps = 1e-6; smooth random fields, and white noise
-reported beside them).  Two more legs time FP64, FP32-MFMA and
-split-BF16x3 on the Euler vortex at N = 4 (512x512x2, K = 524288, 200 steps,
-density error against the FP64 run) and on Maxwell TM at N = 8 (384x384x2,
-K = 294912, random 16-mode Ez, 300 steps, MFMA utilisation).
+Legs (each timed with HIP events on the engine's stream):
+  vortex_64_N3       Euler vortex, K = 8192, N = 3, 100 steps, 3 precisions
+  vortex_512_N4      Euler vortex, K = 524288, N = 4, 200 steps, 3 precisions,
+                     density error against the FP64 run
+  tm_fourier_384_N8  Maxwell TM, K = 294912, N = 8, 300 steps, 3 precisions
+  adjoint            reverse LSERK4 on adjoint_vortex_256_N3 (200 steps,
+                     checkpoint 20) and dJ/dq0 against central differences
+  limiter            apl_limit at N = 4 and N = 1 on K = 524288
+cpu_baseline: the oracle (Apollo's loop restated in C, OpenMP over elements)
+on the host cores, for the headline workload and each leg's workload.
 
-    python bench.py [--gpus 1] [--steps 500] [--warmup 20] [--workload c2]
+    python bench.py [--gpus 1] [--steps 500] [--warmup 20] [--workload NAME]
 """
 import argparse
 import ctypes
@@ -42,78 +45,10 @@ sys.path.insert(0, ROOT)
 METRIC = "DG RHS DOF-updates/s: FP64 plain vs FP32-MFMA and adjoint, at stated L2 error"
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15, "hbm": 8.0e12}
 TRAFFIC = os.path.join(ROOT, "profiles", "hbm_traffic_maxwell_tm_N5_K131072.json")
+HEAD_WORKLOAD = "tm_cavity_256_N5"
 
 
-def workload(name):
-    """Mesh + parameters + initial state of a benchmark workload."""
-    from apollo_amd import engine as E
-    if name == "c2":
-        n, N = 256, 5
-        mesh = E.structured_mesh(n, 0.0, 1.0, 0.0, 1.0, 0.1, 1, False)
-        params = dict(order=N, eqn=E.EQN_MAXWELL_TM3, flux=E.FLUX_UPWIND,
-                      bcs=[(E.BC_MAXWELL_TM, [])])
-        cfl = 0.5 / (2 * N + 1)
-        desc = ("2D Maxwell TM (Ez,Bx,By), unit square, PEC walls (Apollo TM boundary state), "
-                "256x256x2 jittered +-0.1h seed 1, K=131072, N=5, upwind flux, LSERK4")
-        return dict(name=name, mesh=mesh, params=params, cfl=cfl, desc=desc, m=3, analytic="cavity")
-    if name == "c5":
-        n, N = 384, 8
-        mesh = E.structured_mesh(n, 0.0, 1.0, 0.0, 1.0, 0.1, 4, False)
-        params = dict(order=N, eqn=E.EQN_MAXWELL_TM3, flux=E.FLUX_UPWIND,
-                      bcs=[(E.BC_MAXWELL_TM, [])])
-        cfl = 0.5 / (2 * N + 1)
-        desc = ("2D Maxwell TM, 384x384x2 jittered seed 4, K=294912, N=8, upwind flux, "
-                "random 16-mode Ez")
-        return dict(name=name, mesh=mesh, params=params, cfl=cfl, desc=desc, m=3, analytic="fourier")
-    if name == "c3":
-        n, N = 512, 4
-        mesh = E.structured_mesh(n, -10.0, 10.0, -10.0, 10.0, 0.15, 2, True)
-        params = dict(order=N, eqn=E.EQN_EULER4, bcs=[])
-        desc = ("2D Euler isentropic vortex, [-10,10]^2 periodic, 512x512x2 jittered 0.15h seed 2, "
-                "K=524288, N=4, LF flux")
-        return dict(name=name, mesh=mesh, params=params, cfl=0.5, desc=desc, m=4, analytic="vortex",
-                    L=20.0)
-    if name == "c4":
-        n, N = 256, 3
-        mesh = E.structured_mesh(n, -5.0, 5.0, -5.0, 5.0, 0.1, 3, True)
-        params = dict(order=N, eqn=E.EQN_EULER4, bcs=[])
-        desc = ("adjoint: 2D Euler isentropic vortex, [-5,5]^2 periodic, 256x256x2 jittered 0.1h "
-                "seed 3, K=131072, N=3, LF flux, J = int_{r<1} rho^2 at step 200, checkpoint 20")
-        return dict(name=name, mesh=mesh, params=params, cfl=0.5, desc=desc, m=4, analytic="vortex",
-                    L=10.0)
-    if name in ("c6", "c6n1"):
-        # c6n1: the same at N = 1, the order of Apollo's shock decks
-        n, N = 512, (4 if name == "c6" else 1)
-        mesh = E.structured_mesh(n, -10.0, 10.0, -10.0, 10.0, 0.15, 5, False)
-        params = dict(order=N, eqn=E.EQN_EULER5, flux=E.FLUX_HLL, bcs=[(E.BC_EULER_WALL, [])],
-                      limiter=(1.4, 1e-2, 1e-2))
-        desc = ("Hesthaven-Warburton Euler limiter (apl_limit, in place): Apollo's 5-variable "
-                "Euler, [-10,10]^2 with walls, 512x512x2 jittered 0.15h seed 5, K=524288, N=%d, "
-                "isentropic-vortex state" % N)
-        return dict(name=name, mesh=mesh, params=params, cfl=0.5, desc=desc, m=5, analytic="vortex5",
-                    L=20.0)
-    raise SystemExit("unknown workload %s" % name)
-
-
-def initial_state(w, x, y):
-    from apollo_amd import engine as E
-    if w["analytic"] == "cavity":
-        return E.tm_cavity_mode(x, y, 0.0, m=3)
-    if w["analytic"] == "fourier":
-        return E.random_fourier_ez(x, y, 16, 4, m=3)
-    return E.periodic_vortex_state(x, y, 0.0, w["L"], m=4)
-
-
-def exact_state(w, x, y, t):
-    from apollo_amd import engine as E
-    if w["analytic"] == "cavity":
-        return E.tm_cavity_mode(x, y, t, m=3)
-    if w["analytic"] == "vortex":
-        return E.periodic_vortex_state(x, y, t, w["L"], m=4)
-    return None
-
-
-def flop_alg(K, N, Np, Nc, Ng, m):
+def flop_alg(K, Np, Nc, Ng, m):
     """Contraction flops of one RHS of Apollo's cubature form:
     K 2 m (3 Nc Np + 6 Ng Np) (1 FMA = 2 flops)."""
     return K * 2 * m * (3 * Nc * Np + 6 * Ng * Np)
@@ -138,27 +73,29 @@ def bytes_alg_stage(K, Np, m):
     return K * (4 * Np * m * 8 + 136)
 
 
+VARIANTS = {
+    # name: (precision, algorithm)
+    "fp64": (0, 0), "fp32": (1, 0), "bf16x3": (2, 0),
+    "fp64_cub": (0, 1), "fp32_cub": (1, 1), "bf16x3_cub": (2, 1),
+}
+
+
 def engine_on_stream(w, precision, algorithm, torch):
     from apollo_amd import engine as E
-    eng = E.Engine(w["mesh"], E.Params(precision=precision, algorithm=algorithm, **w["params"]))
+    eng = E.Engine(w.mesh(), w.params(precision, algorithm))
     stream = torch.cuda.Stream()
     E.lib().apl_set_stream(eng._h, ctypes.c_void_p(stream.cuda_stream))
     return eng, stream
 
 
-def fixed_dt(w, eng, q0):
-    """Apollo's dt formula, evaluated once from q0 and then held fixed."""
-    if w["analytic"] in ("cavity", "fourier"):
-        return eng.suggested_dt(w["cfl"], 1.0)      # c0 = 1
-    tmp = np.zeros_like(q0)
-    return eng.rhs(0.0, q0, tmp, cfl=w["cfl"])
-
-
-def run_gpu(w, precision, steps, warmup, torch, algorithm=0):
+def run_gpu(w, variant, steps, warmup, torch):
+    """`steps` LSERK4 steps of one variant, timed with HIP events on the
+    engine's stream; returns the timing and the final state."""
+    precision, algorithm = VARIANTS[variant]
     eng, stream = engine_on_stream(w, precision, algorithm, torch)
     g, x, y = eng.geometry()
-    q0 = np.ascontiguousarray(initial_state(w, x, y))
-    dt = fixed_dt(w, eng, q0)
+    q0 = np.ascontiguousarray(w.initial_state(x, y))
+    dt = w.fixed_dt(eng, q0)
     with torch.cuda.stream(stream):
         qd = torch.from_numpy(q0.copy()).cuda()
         # warm-up (graph capture happens here)
@@ -176,18 +113,72 @@ def run_gpu(w, precision, steps, warmup, torch, algorithm=0):
     qT = qd.cpu().numpy()
     info = dict(K=eng.K, N=eng.N, Np=eng.Np, Nc=eng.Nc, Ng=eng.Ng, m=eng.m, dt=dt)
     eng.close()
-    return dict(wall=wall, gpu_ms=gpu_ms, qT=qT, x=x, y=y, info=info, dt=dt)
+    return dict(wall=wall, gpu_ms=gpu_ms, qT=qT, x=x, y=y, info=info, dt=dt, q0=q0)
 
 
-def run_adjoint(torch, steps=200, ckpt=20, ndir=8, eps=1e-6):
-    """Reverse LSERK4 on the adjoint workload: timing and the finite-difference
-    check of dJ/dq0 along `ndir` Gaussian directions."""
-    from apollo_amd import engine as E
-    w = workload("c4")
-    eng, stream = engine_on_stream(w, E.PREC_FP64, E.ALGO_AUTO, torch)
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def variant_figures(w, variant, r, steps):
+    """DOF-updates/s, per-launch time and roofline figures of one run."""
+    info = r["info"]
+    K, N, Np, Nc, Ng, m = (info[k] for k in ("K", "N", "Np", "Nc", "Ng", "m"))
+    lin = w.linear and VARIANTS[variant][1] == 0
+    kern_s = r["gpu_ms"] * 1e-3 / (5 * steps)
+    d = dict(dof_updates_per_s=K * Np * m / kern_s, ms_per_step=r["gpu_ms"] / steps,
+             kernel_us=kern_s * 1e6, form="nodal" if lin else "cubature",
+             tflops_alg=flop_alg(K, Np, Nc, Ng, m) / kern_s / 1e12,
+             tflops_exec=flop_exec(lin, K, N, Np, Nc, Ng, m) / kern_s / 1e12,
+             gbs_alg=bytes_alg_stage(K, Np, m) / kern_s / 1e9)
+    d["hbm_frac_alg"] = d["gbs_alg"] * 1e9 / PEAK["hbm"]
+    pk = PEAK["fp64" if variant.startswith("fp64") else ("fp32" if variant.startswith("fp32") else "bf16")]
+    d["mfma_frac_exec"] = d["tflops_exec"] * 1e12 / pk
+    return d
+
+
+def run_leg(name, variants, torch, steps=None):
+    """A workload timed like the headline one over its own step count: each
+    variant's figures, its speed-up over FP64 and its relative L2 error
+    against the FP64 run (density only for Euler) and the analytic solution
+    where there is one."""
+    from apollo_amd import workloads as W
+    w = W.get(name)
+    steps = w.steps if steps is None else steps
+    res = {}
+    for v in variants:
+        print("bench: %s %s steps=%d" % (name, v, steps), file=sys.stderr, flush=True)
+        res[v] = run_gpu(w, v, steps, 3, torch)
+    sel = (lambda q: q[..., 0]) if w.m == 4 else (lambda q: q)
+    ref = res["fp64"]["qT"]
+    out = {"workload": w.desc, "steps": steps, "dt": res["fp64"]["dt"], "variants": {}}
+    for v in variants:
+        r = res[v]
+        d = variant_figures(w, v, r, steps)
+        if v != "fp64":
+            d["l2_vs_fp64"] = rel_l2(sel(r["qT"]), sel(ref))
+            d["speedup_vs_fp64"] = res["fp64"]["gpu_ms"] / r["gpu_ms"]
+        ex = w.exact_state(r["x"], r["y"], steps * r["dt"])
+        if ex is not None:
+            d["l2_vs_analytic"] = rel_l2(sel(r["qT"]), sel(ex))
+        out["variants"][v] = d
+    if w.m == 4:
+        out["error_field"] = "density"
+    return out
+
+
+def run_adjoint(torch):
+    """Reverse LSERK4 on adjoint_vortex_256_N3: timing, and dJ/dq0 against
+    central finite differences along 8 Gaussian directions of two kinds,
+    smooth random fields and white noise, at eps = 1e-6 (and white noise at
+    1e-7 as well, to show the eps-scaling of its error)."""
+    from apollo_amd import workloads as W
+    w = W.get("adjoint_vortex_256_N3")
+    steps, ckpt = w.steps, w.ckpt
+    eng, stream = engine_on_stream(w, 0, 0, torch)
     g, x, y = eng.geometry()
-    q0 = np.ascontiguousarray(initial_state(w, x, y))
-    dt = fixed_dt(w, eng, q0)
+    q0 = np.ascontiguousarray(w.initial_state(x, y))
+    dt = w.fixed_dt(eng, q0)
     disc = (0, 0.0, 0.0, 1.0)              # density, centre (0, 0), radius 1
     with torch.cuda.stream(stream):
         q0d = torch.from_numpy(q0).cuda()
@@ -220,84 +211,49 @@ def run_adjoint(torch, steps=200, ckpt=20, ndir=8, eps=1e-6):
         torch.cuda.synchronize()
         t_adj = ev[0].elapsed_time(ev[1]) * 1e-3
         t_arhs = ev[2].elapsed_time(ev[3]) * 1e-3 / 100
-        gen = torch.Generator(device="cpu").manual_seed(8)
 
-        def fd_err(d):
+        def fd_err(d, eps):
             fd = (objective(q0d + eps * d) - objective(q0d - eps * d)) / (2 * eps)
             ad = float((grad * d).sum())
             return abs(fd - ad) / abs(ad)
 
-        # white-noise directions: N(0,1) per degree of freedom
-        errs = [fd_err(torch.randn(q0d.shape, generator=gen, dtype=torch.float64).cuda())
-                for _ in range(ndir)]
+        gen = torch.Generator(device="cpu").manual_seed(8)
+        white = [torch.randn(q0d.shape, generator=gen, dtype=torch.float64).cuda() for _ in range(8)]
+        errs6 = [fd_err(d, 1e-6) for d in white]
+        errs7 = [fd_err(d, 1e-7) for d in white]
         # smooth Gaussian directions: 16 random Fourier modes of the periodic
         # domain per variable, N(0,1) amplitudes
         rng = np.random.default_rng(8)
-        L = w["L"]
         errs_smooth = []
-        for _ in range(ndir):
+        for _ in range(8):
             d = np.zeros(q0.shape)
             for v in range(q0.shape[-1]):
                 for _ in range(16):
                     kx, ky = rng.integers(-3, 4, size=2)
                     ph = rng.uniform(0, 2 * np.pi)
-                    d[..., v] += rng.standard_normal() * np.cos(2 * np.pi * (kx * x + ky * y) / L + ph)
-            errs_smooth.append(fd_err(torch.from_numpy(d).cuda()))
-    dofs = eng.K * eng.Np * w["m"]
-    res = dict(workload=w["desc"], steps=steps, ckpt_every=ckpt, dt=dt, J=J,
+                    d[..., v] += rng.standard_normal() * np.cos(2 * np.pi * (kx * x + ky * y) / w.L + ph)
+            errs_smooth.append(fd_err(torch.from_numpy(d).cuda(), 1e-6))
+    dofs = eng.K * eng.Np * w.m
+    res = dict(workload=w.desc, steps=steps, ckpt_every=ckpt, dt=dt, J=J,
                seconds=t_adj,
                adjoint_dof_updates_per_s=dofs * 5 * steps / t_adj,
                adjoint_rhs_us=t_arhs * 1e6,
                adjoint_rhs_dof_updates_per_s=dofs / t_arhs,
-               fd_directions=ndir, fd_eps=eps, fd_max_rel_err=max(errs_smooth),
-               fd_max_rel_err_white_noise=max(errs),
-               fd_note="fd_max_rel_err: smooth Gaussian random-field directions (16 Fourier modes "
-                       "per variable, N(0,1) amplitudes); white-noise directions (N(0,1) per degree "
-                       "of freedom) move the two traces of a face independently by ~eps, which "
-                       "straddles the Lax-Friedrichs max(c-, c+) at part of the face points, so "
-                       "central differences there see the average of the two branches",
+               fd_directions=8, fd_eps=1e-6,
+               fd_max_rel_err_smooth=max(errs_smooth),
+               fd_max_rel_err_white_noise=max(errs6),
+               fd_max_rel_err_white_noise_eps1e7=max(errs7),
+               fd_note="smooth: Gaussian random fields (16 Fourier modes per variable, N(0,1) "
+                       "amplitudes); white noise: N(0,1) per degree of freedom.  The LF speed "
+                       "max(|u-|+c-, |u+|+c+) has a kink where the traces' speeds cross; white noise "
+                       "moves the two traces of a face independently by ~eps, so a fraction ~eps of "
+                       "the face points cross it and the central-difference error falls like eps "
+                       "(compare eps 1e-6 and 1e-7); smooth directions move both traces together",
                note="adjoint DOF-updates/s counts the 5 x steps adjoint RHS evaluations of the "
                     "reverse sweep over the whole reverse LSERK4 call (forward sweep, segment "
                     "recompute and stage rebuild included)")
     eng.close()
     return res
-
-
-def run_leg(name, steps, variants, torch):
-    """A secondary workload timed like the headline one: DOF-updates/s and
-    the L2 error of each variant against the FP64 run after `steps` steps
-    (density only for Euler)."""
-    w = workload(name)
-    res = {}
-    for vname, prec, algo in variants:
-        print("bench: %s %s steps=%d" % (name, vname, steps), file=sys.stderr, flush=True)
-        res[vname] = run_gpu(w, prec, steps, 5, torch, algo)
-    info = res["fp64"]["info"]
-    K, N, Np, Nc, Ng, m = (info[k] for k in ("K", "N", "Np", "Nc", "Ng", "m"))
-    ref = res["fp64"]["qT"]
-    sel = (lambda q: q[..., 0]) if m == 4 else (lambda q: q)
-    out = {"workload": w["desc"], "steps": steps, "dt": info["dt"], "variants": {}}
-    for vname, prec, algo in variants:
-        r = res[vname]
-        kern_s = r["gpu_ms"] * 1e-3 / (5 * steps)
-        lin = m == 3 and algo == 0
-        d = dict(dof_updates_per_s=K * Np * m / kern_s, kernel_us=kern_s * 1e6,
-                 form="nodal" if lin else "cubature",
-                 tflops_alg=flop_alg(K, N, Np, Nc, Ng, m) / kern_s / 1e12,
-                 tflops_exec=flop_exec(lin, K, N, Np, Nc, Ng, m) / kern_s / 1e12,
-                 gbs_alg=bytes_alg_stage(K, Np, m) / kern_s / 1e9)
-        d["hbm_frac_alg"] = d["gbs_alg"] * 1e9 / PEAK["hbm"]
-        if vname != "fp64":
-            d["l2_vs_fp64"] = float(np.linalg.norm((sel(r["qT"]) - sel(ref)).ravel())
-                                    / np.linalg.norm(sel(ref).ravel()))
-        ex = exact_state(w, r["x"], r["y"], steps * r["dt"])
-        if ex is not None:
-            d["l2_vs_analytic"] = float(np.linalg.norm((sel(r["qT"]) - sel(ex)).ravel())
-                                        / np.linalg.norm(sel(ex).ravel()))
-        pk = PEAK["fp64" if vname == "fp64" else ("fp32" if vname == "fp32" else "bf16")]
-        d["mfma_frac_exec"] = d["tflops_exec"] * 1e12 / pk
-        out["variants"][vname] = d
-    return out
 
 
 def bytes_alg_limit(K, Np, m):
@@ -307,13 +263,17 @@ def bytes_alg_limit(K, Np, m):
     return K * (2 * Np * m * 8 + 140)
 
 
-def run_limiter(torch, reps=50, name="c6"):
-    """Time apl_limit in place on a limiter workload (c6: N = 4, c6n1:
-    N = 1) with HIP events on the engine's stream; report applications/s,
-    DOF/s and the algorithmic-byte rate against the HBM peak."""
+def run_limiter(torch, N, reps=50):
+    """apl_limit in place on Apollo's 5-variable Euler, [-10,10]^2 with walls,
+    512x512x2 jittered 0.15h seed 5, K=524288, isentropic-vortex state, at
+    order N: applications/s and the algorithmic-byte rate."""
     from apollo_amd import engine as E
-    w = workload(name)
-    eng, stream = engine_on_stream(w, E.PREC_FP64, E.ALGO_AUTO, torch)
+    mesh = E.structured_mesh(512, -10.0, 10.0, -10.0, 10.0, 0.15, 5, False)
+    params = E.Params(N, E.EQN_EULER5, flux=E.FLUX_HLL, bcs=[(E.BC_EULER_WALL, [])],
+                      limiter=(1.4, 1e-2, 1e-2))
+    eng = E.Engine(mesh, params)
+    stream = torch.cuda.Stream()
+    E.lib().apl_set_stream(eng._h, ctypes.c_void_p(stream.cuda_stream))
     g, x, y = eng.geometry()
     q0 = np.ascontiguousarray(E.vortex_state(x, y, 0.0, m=5))
     with torch.cuda.stream(stream):
@@ -331,8 +291,10 @@ def run_limiter(torch, reps=50, name="c6"):
     K, Np, m = eng.K, eng.Np, eng.m
     eng.close()
     ach = bytes_alg_limit(K, Np, m) / sec
-    return {"workload": w["desc"], "reps": reps, "us_per_application": sec * 1e6,
-            "dof_per_s": K * Np * m / sec,
+    return {"workload": "Hesthaven-Warburton Euler limiter (apl_limit, in place): Apollo's "
+                        "5-variable Euler, [-10,10]^2 with walls, 512x512x2 jittered 0.15h seed 5, "
+                        "K=524288, N=%d, isentropic-vortex state" % N,
+            "reps": reps, "us_per_application": sec * 1e6, "dof_per_s": K * Np * m / sec,
             "roofline": {"bound": "hbm", "achieved": ach / 1e9, "peak": PEAK["hbm"] / 1e9,
                          "unit": "GB/s", "frac": ach / PEAK["hbm"],
                          "bytes_alg_per_element": bytes_alg_limit(1, Np, m)},
@@ -340,35 +302,7 @@ def run_limiter(torch, reps=50, name="c6"):
                     "flag check per call"}
 
 
-def cpu_baseline(w, nthreads, budget_s=15.0):
-    """Oracle RHS (Apollo's loop restated in C, Apollo's 8-variable Maxwell /
-    5-variable Euler layout) on the host cores, on the full mesh."""
-    from oracle import oracle as O
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import cases
-    from apollo_amd import engine as E
-    params = E.Params(**w["params"])
-    case = cases.Case(w["mesh"], params)
-    q = cases.to_oracle(params.eqn, initial_state(w, case.omesh.x, case.omesh.y))
-    O.rhs(case.omesh, case.ophys, 0.0, q, nthreads=nthreads)       # warm-up
-    t0 = time.perf_counter()
-    O.rhs(case.omesh, case.ophys, 0.0, q, nthreads=nthreads)
-    t1 = time.perf_counter() - t0
-    reps = int(max(3, min(20, budget_s / max(t1, 1e-6))))
-    for _ in range(2):
-        O.rhs(case.omesh, case.ophys, 0.0, q, nthreads=nthreads)
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        O.rhs(case.omesh, case.ophys, 0.0, q, nthreads=nthreads)
-    el = time.perf_counter() - t0
-    dofs = case.mesh.K * case.ops.Np * w["m"]
-    # one core: the same RHS, fewer repetitions
-    O.rhs(case.omesh, case.ophys, 0.0, q, nthreads=1)
-    reps1 = 3
-    t0 = time.perf_counter()
-    for _ in range(reps1):
-        O.rhs(case.omesh, case.ophys, 0.0, q, nthreads=1)
-    el1 = time.perf_counter() - t0
+def _cpu_host():
     model = ""
     try:
         for ln in open("/proc/cpuinfo"):
@@ -378,13 +312,76 @@ def cpu_baseline(w, nthreads, budget_s=15.0):
     except OSError:
         pass
     usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cores": usable}
+
+
+def _oracle_case(w):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import cases
+    case = cases.Case(w.mesh(), w.params())
+    q = cases.to_oracle(w.eqn, w.initial_state(case.omesh.x, case.omesh.y))
+    return case, q
+
+
+def cpu_rhs_rate(name, nthreads, budget_s, max_reps=20, min_reps=1):
+    """Oracle RHS evaluations of the full mesh (Apollo's 8-variable Maxwell /
+    5-variable Euler layout), after one warm-up, timed for about budget_s."""
+    from apollo_amd import workloads as W
+    from oracle import oracle as O
+    w = W.get(name)
+    case, q = _oracle_case(w)
+    t0 = time.perf_counter()
+    O.rhs(case.omesh, case.ophys, 0.0, q, nthreads=nthreads)          # warm-up
+    t1 = time.perf_counter() - t0
+    reps = int(max(min_reps, min(max_reps, budget_s / max(t1, 1e-6))))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        O.rhs(case.omesh, case.ophys, 0.0, q, nthreads=nthreads)
+    el = time.perf_counter() - t0
+    dofs = case.mesh.K * case.ops.Np * w.m
     return dict(value=dofs * reps / el, unit="DOF-updates/s", cores=nthreads, kind="port",
-                sample="%d RHS evaluations of the full mesh (K=%d, Apollo's %d-variable layout) "
-                       "after 3 warm-ups, OpenMP over elements" % (reps, case.mesh.K, q.shape[-1]),
-                rhs_per_s=reps / el,
-                single_core={"value": dofs * reps1 / el1, "rhs_per_s": reps1 / el1,
-                             "sample": "%d RHS evaluations, 1 thread" % reps1},
-                host={"cpu_model": model, "nproc": os.cpu_count(), "usable_cores": usable})
+                sample="%d RHS evaluation(s) of the full mesh (K=%d, Apollo's %d-variable layout) "
+                       "after 1 warm-up, %d OpenMP thread(s) over elements"
+                       % (reps, case.mesh.K, q.shape[-1], nthreads),
+                rhs_per_s=reps / el)
+
+
+def cpu_lserk4_run(name, nthreads, steps):
+    """The oracle's LSERK4 on the full mesh over `steps` steps at the
+    workload's fixed dt (for vortex_64_N3 the whole 100-step run)."""
+    from apollo_amd import workloads as W
+    from oracle import oracle as O
+    w = W.get(name)
+    case, q = _oracle_case(w)
+    _, ms = O.rhs(case.omesh, case.ophys, 0.0, q, nthreads=nthreads)
+    dt = O.suggested_dt(case.omesh, w.cfl, ms)
+    t0 = time.perf_counter()
+    O.lserk4(case.omesh, case.ophys, 0.0, dt, steps, q, nthreads=nthreads)
+    el = time.perf_counter() - t0
+    dofs = case.mesh.K * case.ops.Np * w.m
+    return dict(value=dofs * 5 * steps / el, unit="DOF-updates/s", cores=nthreads, kind="port",
+                sample="%d LSERK4 steps (%d RHS evaluations) of the full mesh (K=%d, Apollo's "
+                       "%d-variable layout), %d OpenMP thread(s)"
+                       % (steps, 5 * steps, case.mesh.K, q.shape[-1], nthreads),
+                seconds=el, steps=steps)
+
+
+def cpu_baselines(nthreads):
+    """The headline cpu_baseline and one per leg workload, each bounded to a
+    few seconds of host time."""
+    print("bench: cpu baseline, %d threads" % nthreads, file=sys.stderr, flush=True)
+    head = cpu_rhs_rate(HEAD_WORKLOAD, nthreads, budget_s=12.0, min_reps=3)
+    head["single_core"] = cpu_rhs_rate(HEAD_WORKLOAD, 1, budget_s=6.0, max_reps=3)
+    head["host"] = _cpu_host()
+    legs = {
+        "vortex_64_N3": cpu_lserk4_run("vortex_64_N3", nthreads, 100),
+        "vortex_64_N3_single_core": cpu_lserk4_run("vortex_64_N3", 1, 10),
+        "vortex_512_N4": cpu_rhs_rate("vortex_512_N4", nthreads, budget_s=4.0, max_reps=4),
+        "adjoint_vortex_256_N3": cpu_rhs_rate("adjoint_vortex_256_N3", nthreads, budget_s=3.0,
+                                              max_reps=8),
+        "tm_fourier_384_N8": cpu_rhs_rate("tm_fourier_384_N8", nthreads, budget_s=5.0, max_reps=3),
+    }
+    return head, legs
 
 
 def measured_traffic(key):
@@ -417,18 +414,29 @@ def calibrate_copy(mib, reps=10):
                       "us_per_copy": s * 1e6, "gbs": 2 * n * 8 / s / 1e9}), flush=True)
 
 
+def kernel_name(w, variant):
+    from apollo_amd import engine as E
+    N = w.order
+    lin = w.linear and VARIANTS[variant][1] == 0
+    ty = {"fp64": "double", "fp32": "float", "bf16x3": "apl::Bf16x3"}[variant.split("_")[0]]
+    return "%s<%d,%d,%d,%s>" % ("rhs_linear_kernel" if lin else "rhs_kernel", N, w.eqn, w.flux, ty)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default=HEAD_WORKLOAD)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-legs", action="store_true",
-                    help="skip the Euler N=4 and Maxwell N=8 workloads")
+                    help="skip the vortex, N=8 Maxwell and limiter legs")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the fixed 500-step accuracy run of the headline workload")
     ap.add_argument("--only", default="",
                     help="time one variant only (fp64, fp32, bf16x3, fp64_cub, fp32_cub, bf16x3_cub)")
+    ap.add_argument("--limiter-only", action="store_true", help="time the limiter legs only")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run a plain MIB-MiB device copy instead (counter calibration)")
     args = ap.parse_args()
@@ -440,57 +448,29 @@ def main():
               "documented, not built); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
-    from apollo_amd import engine as E
+    from apollo_amd import workloads as W
     assert torch.cuda.is_available()
-    if args.workload == "c6":
-        print(json.dumps({"limiter_c6": run_limiter(torch),
-                          "limiter_c6_n1": run_limiter(torch, name="c6n1")}), flush=True)
+    if args.limiter_only:
+        print(json.dumps({"limiter_N4": run_limiter(torch, 4), "limiter_N1": run_limiter(torch, 1)}),
+              flush=True)
         return
-    w = workload(args.workload)
-    linear_ok = w["m"] == 3
+    w = W.get(args.workload)
+    names = ["fp64", "fp32", "bf16x3"] + (["fp64_cub", "fp32_cub", "bf16x3_cub"] if w.linear else [])
+    if args.only:
+        names = [args.only]
     res = {}
-    variants = [("fp64", E.PREC_FP64, E.ALGO_AUTO), ("fp32", E.PREC_FP32_MFMA, E.ALGO_AUTO),
-                ("bf16x3", E.PREC_BF16X3, E.ALGO_AUTO)]
-    if linear_ok:
-        # Maxwell: the default is the nodal form of the linear flux; also time
-        # Apollo's over-integrated cubature form
-        variants += [("fp64_cub", E.PREC_FP64, E.ALGO_CUBATURE),
-                     ("fp32_cub", E.PREC_FP32_MFMA, E.ALGO_CUBATURE),
-                     ("bf16x3_cub", E.PREC_BF16X3, E.ALGO_CUBATURE)]
-    for name, prec, algo in variants:
-        if args.only and args.only != name:
-            continue
-        print("bench: %s %s steps=%d" % (args.workload, name, args.steps), file=sys.stderr, flush=True)
-        r = run_gpu(w, prec, args.steps, args.warmup, torch, algo)
-        r["linear"] = linear_ok and algo == E.ALGO_AUTO
-        res[name] = r
+    for v in names:
+        print("bench: %s %s steps=%d" % (args.workload, v, args.steps), file=sys.stderr, flush=True)
+        res[v] = run_gpu(w, v, args.steps, args.warmup, torch)
+    out = {v: variant_figures(w, v, r, args.steps) for v, r in res.items()}
+    for v, r in res.items():
+        if v != "fp64" and "fp64" in res:
+            out[v]["speedup_vs_fp64"] = res["fp64"]["gpu_ms"] / r["gpu_ms"]
+            out[v]["l2_vs_fp64_at_timed_steps"] = rel_l2(r["qT"], res["fp64"]["qT"])
     info = next(iter(res.values()))["info"]
     K, N, Np, Nc, Ng, m = (info[k] for k in ("K", "N", "Np", "Nc", "Ng", "m"))
-    rhs_evals = 5 * args.steps
-    out = {}
-    for name, r in res.items():
-        ms = r["gpu_ms"]
-        kern_s = ms * 1e-3 / rhs_evals
-        dofs = K * Np * m * rhs_evals / (ms * 1e-3)
-        T = args.steps * r["dt"]
-        ex = exact_state(w, r["x"], r["y"], T)
-        err_exact = None
-        if ex is not None:
-            err_exact = float(np.linalg.norm((r["qT"] - ex).ravel()) / np.linalg.norm(ex.ravel()))
-        out[name] = dict(dof_updates_per_s=dofs, ms_per_step=ms / args.steps,
-                         kernel_us=kern_s * 1e6, l2_vs_analytic=err_exact, T=T,
-                         form="nodal" if r["linear"] else "cubature",
-                         tflops_alg=flop_alg(K, N, Np, Nc, Ng, m) / kern_s / 1e12,
-                         tflops_exec=flop_exec(r["linear"], K, N, Np, Nc, Ng, m) / kern_s / 1e12,
-                         gbs_alg=bytes_alg_stage(K, Np, m) / kern_s / 1e9)
-        if name != "fp64" and "fp64" in res:
-            a, b = r["qT"], res["fp64"]["qT"]
-            out[name]["l2_vs_fp64"] = float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
-    head = "fp32" if "fp32" in out else next(iter(out))
+    head = "fp64" if "fp64" in out else names[0]
     h = out[head]
-    kname = "%s<%d,%d,%d,%s>" % ("rhs_linear_kernel" if res[head]["linear"] else "rhs_kernel", N,
-                                 w["params"]["eqn"], w["params"].get("flux", 0),
-                                 "double" if head.startswith("fp64") else "float")
     line = {
         "metric": METRIC,
         "value": h["dof_updates_per_s"],
@@ -502,60 +482,92 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if head.startswith("fp64") else "f32",
+        "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3"}[head.split("_")[0]],
         "data": "synthetic",
-        "config": {"workload": w["desc"], "K": K, "N": N, "Np": Np, "Nc": Nc, "Ng": Ng,
-                   "m": m, "dt": info["dt"], "rhs_evals_per_step": 5,
-                   "precision": "FP32-MFMA contractions, FP64 state/geometry/stage update"
-                   if head == "fp32" else head},
+        "config": {"workload": w.desc, "name": w.name, "K": K, "N": N, "Np": Np, "Nc": Nc,
+                   "Ng": Ng, "m": m, "dt": info["dt"], "rhs_evals_per_step": 5,
+                   "precision": "FP64 everywhere (the reference's precision); nodal form of the "
+                                "linear flux, identical to Apollo's cubature form to rounding"
+                   if head == "fp64" else head},
         "variants": out,
-        "speedup_fp32_vs_fp64": (out["fp32"]["dof_updates_per_s"] / out["fp64"]["dof_updates_per_s"])
-        if ("fp32" in out and "fp64" in out) else None,
     }
-    # Roofline of the dominant kernel (the head variant's fused RHS + stage):
-    # the nodal form executes ~2.5 flop per algorithmic byte, far below the
-    # MFMA ridge, so its bound is HBM; the cubature form is reported against
-    # its MFMA peak beside it.
+    # Roofline of the dominant kernel (the head variant's fused RHS + stage).
+    # The nodal form executes ~2.5 flop per algorithmic byte, far below the
+    # FP64 and FP32 ridges (9.8 and 19.7 flop/B), so its bound is HBM; the
+    # cubature form is reported against its MFMA peak beside it.
     bpl = bytes_alg_stage(K, Np, m)
-    if res[head]["linear"]:
+    lin = w.linear and VARIANTS[head][1] == 0
+    kname = kernel_name(w, head)
+    if lin:
         ach = bpl / (h["kernel_us"] * 1e-6)
         line["roofline"] = {"bound": "hbm", "achieved": ach / 1e9, "peak": PEAK["hbm"] / 1e9,
                             "unit": "GB/s", "frac": ach / PEAK["hbm"],
-                            "traffic": measured_traffic("%s/%s" % (w["name"], head)),
+                            "traffic": measured_traffic("c2/%s" % head),
                             "kernel": kname + " (fused RHS + LSERK4 stage), avg over %d launches"
-                                      % rhs_evals,
+                                      % (5 * args.steps),
                             "bytes_per_launch": bpl}
     else:
         pk = PEAK["fp64" if head.startswith("fp64") else "fp32"]
         ach = h["tflops_alg"] * 1e12
         line["roofline"] = {"bound": "mfma", "achieved": ach / 1e12, "peak": pk / 1e12,
-                            "unit": "TFLOP/s", "frac": ach / pk,
-                            "traffic": measured_traffic("%s/%s" % (w["name"], head)),
+                            "unit": "TFLOP/s", "frac": ach / pk, "traffic": None,
                             "kernel": kname + " (fused RHS + LSERK4 stage), avg over %d launches"
-                                      % rhs_evals,
-                            "flop_per_launch": flop_alg(K, N, Np, Nc, Ng, m)}
-    for cub in ("fp64_cub", "fp32_cub", "bf16x3_cub"):
-        if cub in out:
-            pk = PEAK["fp64" if cub == "fp64_cub" else ("fp32" if cub == "fp32_cub" else "bf16")]
-            ach = out[cub]["tflops_exec"] * 1e12
-            line.setdefault("roofline_cubature", {})[cub] = {
-                "bound": "mfma", "achieved_exec": ach / 1e12, "peak": pk / 1e12, "unit": "TFLOP/s",
-                "frac": ach / pk, "traffic": measured_traffic("%s/%s" % (w["name"], cub))}
-    if args.workload == "c2" and not args.no_adjoint and not args.only:
-        print("bench: adjoint (c4)", file=sys.stderr, flush=True)
+                                      % (5 * args.steps),
+                            "flop_per_launch": flop_alg(K, Np, Nc, Ng, m)}
+    line["roofline_variants"] = {}
+    for v in out:
+        if v == head:
+            continue
+        lv = w.linear and VARIANTS[v][1] == 0
+        if lv:
+            line["roofline_variants"][v] = {
+                "bound": "hbm", "achieved": out[v]["gbs_alg"], "peak": PEAK["hbm"] / 1e9,
+                "unit": "GB/s", "frac": out[v]["hbm_frac_alg"],
+                "traffic": measured_traffic("c2/%s" % v), "kernel": kernel_name(w, v)}
+        else:
+            pk = PEAK["fp64" if v.startswith("fp64") else ("fp32" if v.startswith("fp32") else "bf16")]
+            line["roofline_variants"][v] = {
+                "bound": "mfma", "achieved_exec": out[v]["tflops_exec"], "peak": pk / 1e12,
+                "unit": "TFLOP/s", "frac": out[v]["tflops_exec"] * 1e12 / pk,
+                "traffic": measured_traffic("c2/%s" % v), "kernel": kernel_name(w, v)}
+    if "fp32" in out and "fp64" in out:
+        line["speedup_fp32_vs_fp64"] = out["fp32"]["speedup_vs_fp64"]
+    if not args.no_accuracy and not args.only:
+        # the headline workload's full 500-step run, whatever --steps is:
+        # errors against the analytic cavity mode and the FP64 run
+        acc = {}
+        runs = {}
+        for v in ("fp64", "fp32", "bf16x3", "fp64_cub"):
+            if v not in VARIANTS or (v.endswith("_cub") and not w.linear):
+                continue
+            print("bench: accuracy %s %s steps=%d" % (w.name, v, w.steps), file=sys.stderr, flush=True)
+            runs[v] = run_gpu(w, v, w.steps, 1, torch)
+        for v, r in runs.items():
+            T = w.steps * r["dt"]
+            ex = w.exact_state(r["x"], r["y"], T)
+            acc[v] = {"l2_vs_analytic": rel_l2(r["qT"], ex) if ex is not None else None,
+                      "l2_vs_fp64": rel_l2(r["qT"], runs["fp64"]["qT"]) if v != "fp64" else 0.0}
+        line["accuracy"] = {"steps": w.steps, "T": w.steps * runs["fp64"]["dt"], "variants": acc,
+                            "note": "relative L2 over all variables after the workload's full "
+                                    "step count, independent of --steps"}
+    if not args.no_adjoint and not args.only:
+        print("bench: adjoint", file=sys.stderr, flush=True)
         line["adjoint"] = run_adjoint(torch)
-    if args.workload == "c2" and not args.no_legs and not args.only:
-        three = [("fp64", E.PREC_FP64, E.ALGO_AUTO), ("fp32", E.PREC_FP32_MFMA, E.ALGO_AUTO),
-                 ("bf16x3", E.PREC_BF16X3, E.ALGO_AUTO)]
-        line["euler_c3"] = run_leg("c3", 200, three, torch)
-        line["maxwell_c5"] = run_leg("c5", 300, three, torch)
-        print("bench: limiter (c6)", file=sys.stderr, flush=True)
-        line["limiter_c6"] = run_limiter(torch)
-        line["limiter_c6_n1"] = run_limiter(torch, name="c6n1")
+    if not args.no_legs and not args.only:
+        three = ["fp64", "fp32", "bf16x3"]
+        line["legs"] = {
+            "vortex_64_N3": run_leg("vortex_64_N3", three, torch),
+            "vortex_512_N4": run_leg("vortex_512_N4", three, torch),
+            "tm_fourier_384_N8": run_leg("tm_fourier_384_N8", three, torch),
+        }
+        print("bench: limiter", file=sys.stderr, flush=True)
+        line["legs"]["limiter_N4"] = run_limiter(torch, 4)
+        line["legs"]["limiter_N1"] = run_limiter(torch, 1)
     if not args.no_cpu_baseline:
-        nth = min(16, os.cpu_count() or 1)
-        print("bench: cpu baseline, %d threads" % nth, file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(w, nth)
+        nth = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        head_cpu, legs_cpu = cpu_baselines(nth)
+        line["cpu_baseline"] = head_cpu
+        line["cpu_baseline_legs"] = legs_cpu
     print(json.dumps(line), flush=True)
 
 

@@ -1422,19 +1422,26 @@ static void bc_vjp(const orc_phys *ph, int bcnum, const double *nx, const double
   }
 }
 
-/* Transpose of rhs_element's linearisation, scattering neighbour terms. */
+/* Transpose of rhs_element's linearisation.  Each element's own terms are
+ * computed in parallel; the adjoint traces it hands to its neighbours (qPb
+ * per face point) are kept and scattered afterwards in element order, so
+ * the result does not depend on the thread count. */
 int orc_adjoint_rhs(const orc_ops *o, const orc_phys *ph, int K, const double *xall,
                     const double *yall, const int *etetf, double t, const double *u,
                     const double *lam, double *out) {
   const int Np = o->Np, Nc = o->Nc, Ng = o->Ng, m = orc_meqn(ph);
   size_t nst = (size_t)K * Np * m;
   memset(out, 0, sizeof(double) * nst);
-  int rc;
+  double *pbuf = ALLOC(double, (size_t)K * 3 * Ng * m);
+  if (!pbuf) return 10;
+  int err = 0;
+#pragma omp parallel for schedule(static) reduction(| : err)
   for (int k = 0; k < K; k++) {
     const double *xcoord = xall + (size_t)k * Np, *ycoord = yall + (size_t)k * Np;
     const double *q = u + (size_t)k * Np * m;
     double geo[14];
-    if ((rc = orc_elem_geom(o, xcoord, ycoord, geo))) return rc;
+    int rc;
+    if ((rc = orc_elem_geom(o, xcoord, ycoord, geo))) { err |= rc; continue; }
     const double *normals = geo + 5;
     double a[45 * 8], lt[45 * 8];
     for (int i = 0; i < Np * m; i++) lt[i] = lam[(size_t)k * Np * m + i] / geo[4];
@@ -1504,11 +1511,8 @@ int orc_adjoint_rhs(const orc_ops *o, const orc_phys *ph, int K, const double *x
       if (edgeNum < 0) {
         bc_vjp(ph, abs(edgeNum), normals + 3 * edge, qPb, qMb);
       } else {
-        /* neighbour: row edgeNum*Ng + Ng-1-g of If applied to q_nbr */
-        int nb_e = connect[2 * edge], row = edgeNum * Ng + Ng - 1 - g;
-        double *on = out + (size_t)nb_e * Np * m;
-        for (int j = 0; j < Np; j++)
-          for (int c = 0; c < m; c++) on[j * m + c] += o->If[row * Np + j] * qPb[c];
+        /* the neighbour's share, scattered below */
+        for (int c = 0; c < m; c++) pbuf[((size_t)k * 3 * Ng + kk) * m + c] = qPb[c];
       }
       for (int c = 0; c < m; c++) QMb[kk * m + c] = qMb[c];
     }
@@ -1519,7 +1523,21 @@ int orc_adjoint_rhs(const orc_ops *o, const orc_phys *ph, int K, const double *x
         ok[j * m + c] += s;
       }
   }
-  return 0;
+  /* neighbour terms: row edgeNum*Ng + Ng-1-g of If applied to q_nbr */
+  for (int k = 0; k < K && !err; k++) {
+    const int *connect = etetf + 6 * (size_t)k;
+    for (int kk = 0; kk < 3 * Ng; kk++) {
+      int edge = kk / Ng, g = kk % Ng, edgeNum = connect[2 * edge + 1];
+      if (edgeNum < 0) continue;
+      int nb_e = connect[2 * edge], row = edgeNum * Ng + Ng - 1 - g;
+      const double *qPb = pbuf + ((size_t)k * 3 * Ng + kk) * m;
+      double *on = out + (size_t)nb_e * Np * m;
+      for (int j = 0; j < Np; j++)
+        for (int c = 0; c < m; c++) on[j * m + c] += o->If[row * Np + j] * qPb[c];
+    }
+  }
+  free(pbuf);
+  return err;
 }
 
 int orc_lserk4_adjoint(const orc_ops *o, const orc_phys *ph, int K, const double *x,

@@ -90,6 +90,28 @@ def test_host_connectivity_matches_oracle(periodic):
         assert (a < 0).sum() == 2 * 4 * 9     # 36 boundary faces, two entries each
 
 
+@pytest.mark.parametrize("name", ["vortex_64_N3", "tm_cavity_256_N5", "vortex_512_N4",
+                                  "adjoint_vortex_256_N3", "tm_fourier_384_N8"])
+def test_workload_meshes_full_size(name):
+    """Each benchmark workload's mesh at full size: K, the boundary-face
+    count, and the engine's FacePair bit for bit against the oracle's."""
+    from apollo_amd import workloads as W
+    w = W.get(name)
+    mesh = w.mesh()
+    assert mesh.K == w.K == 2 * w.n * w.n
+    et = E.host_connectivity(mesh)
+    assert np.array_equal(et, O.facepair(mesh.EToV, mesh.Nv, mesh.vert_id, mesh.face_tag))
+    nb = int((et[:, 1::2] < 0).sum())
+    assert nb == (0 if w.periodic else 4 * w.n)
+    # interior vertices are jittered by at most jitter * h, boundary ones not
+    h = w.L / w.n
+    i, j = np.meshgrid(np.arange(w.n + 1), np.arange(w.n + 1), indexing="xy")
+    dx = mesh.VX - (w.box[0] + h * i.ravel())
+    assert np.abs(dx).max() <= w.jitter * h * (1 + 1e-12) and np.abs(dx).max() > 0.9 * w.jitter * h
+    edge = (i.ravel() == 0) | (i.ravel() == w.n) | (j.ravel() == 0) | (j.ravel() == w.n)
+    assert np.all(dx[edge] == 0.0)
+
+
 @pytest.mark.parametrize("name", ["advection_test.msh", "small_square_4bc.msh",
                                   "vortex_grid.msh", "tm_square.msh", "ffs_grid.msh"])
 def test_host_connectivity_gmsh_fixtures(name):

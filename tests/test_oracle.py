@@ -656,3 +656,132 @@ def test_lserk4_adjoint_against_finite_differences():
         fd = (J(q0 + eps * d)[0] - J(q0 - eps * d)[0]) / (2 * eps)
         ad = np.sum(g0 * d)
         assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad)
+
+
+# --- wave speeds and the TM upwind flux, pinned against what the physics fixes ---
+
+def _euler_flux_3d(q, g):
+    """Textbook x- and y-fluxes of the 3D-momentum Euler system
+    (rho, rho u, rho v, rho w, E)."""
+    rho, u, v, w, E = q[0], q[1] / q[0], q[2] / q[0], q[3] / q[0], q[4]
+    p = (g - 1) * (E - 0.5 * rho * (u * u + v * v + w * w))
+    F = np.array([rho * u, rho * u * u + p, rho * u * v, rho * u * w, (E + p) * u])
+    G = np.array([rho * v, rho * v * u, rho * v * v + p, rho * v * w, (E + p) * v])
+    return F, G
+
+
+def _euler_speed(q, g):
+    """|u| + c with |u| over all three velocity components."""
+    rho, u, v, w = q[0], q[1] / q[0], q[2] / q[0], q[3] / q[0]
+    p = (g - 1) * (q[4] - 0.5 * rho * (u * u + v * v + w * w))
+    return math.sqrt(u * u + v * v + w * w) + math.sqrt(g * p / rho)
+
+
+def _prim_to_cons(rho, u, v, w, p, g):
+    return np.array([rho, rho * u, rho * v, rho * w, p / (g - 1) + 0.5 * rho * (u * u + v * v + w * w)])
+
+
+def test_euler_lf_wave_speed():
+    """Lax-Friedrichs (wxeulereqn.cc:1088-1117): F* = (n.(F- + F+) + lam
+    (q- - q+)) / 2 with lam = max over both traces of |u| + c, |u| taken over
+    all three velocity components.  The cases are chosen so that the
+    plausible variants give different numbers: the normal speed |u.n| + c
+    (tangential flow), |u| without w (large w), the minus trace's speed
+    alone (faster plus trace), and c with p in place of gamma p."""
+    rng = np.random.default_rng(21)
+    cases = [
+        # tangential flow along a face with normal (1, 0)
+        ((1.0, 0.0, 2.0, 0.0, 1.0), (1.1, 0.0, 2.1, 0.0, 1.2), (1.0, 0.0)),
+        # large w, small in-plane velocity
+        ((1.0, 0.1, 0.0, 3.0, 1.0), (0.9, 0.1, 0.1, 2.5, 1.1), (0.6, 0.8)),
+        # the plus trace is the faster one
+        ((1.0, 0.1, 0.1, 0.0, 1.0), (0.5, 1.5, -1.0, 0.2, 2.0), (0.0, 1.0)),
+    ]
+    for _ in range(30):
+        a = (0.5 + rng.random(), *(rng.standard_normal(3) * 0.8), 0.5 + rng.random())
+        b = (0.5 + rng.random(), *(rng.standard_normal(3) * 0.8), 0.5 + rng.random())
+        th = rng.random() * 2 * np.pi
+        cases.append((a, b, (math.cos(th), math.sin(th))))
+    for g in (1.4, 5.0 / 3.0):
+        ph = O.phys(O.ORC_EULER, gamma=g)
+        for a, b, n in cases:
+            qa, qb, n = _prim_to_cons(*a, g), _prim_to_cons(*b, g), np.array(n)
+            lam = max(_euler_speed(qa, g), _euler_speed(qb, g))
+            Fa, Ga = _euler_flux_3d(qa, g)
+            Fb, Gb = _euler_flux_3d(qb, g)
+            want = 0.5 * (n[0] * (Fa + Fb) + n[1] * (Ga + Gb) + lam * (qa - qb))
+            got, lam_o = O.numflux(ph, n, qa, qb)
+            assert abs(lam_o - lam) <= 1e-14 * lam
+            assert np.abs(got - want).max() <= 1e-13 * max(1.0, np.abs(want).max())
+    # the first case separates |u| + c from |u.n| + c by the full tangential
+    # speed: the dissipation term is that much larger
+    a, b, n = cases[0]
+    qa, qb = _prim_to_cons(*a, 1.4), _prim_to_cons(*b, 1.4)
+    normal = max(abs(q[1] / q[0]) + math.sqrt(1.4 * 0.4 * (q[4] - 0.5 * (q[1] ** 2 + q[2] ** 2) / q[0]) / q[0])
+                 for q in (qa, qb))
+    assert max(_euler_speed(qa, 1.4), _euler_speed(qb, 1.4)) > normal + 1.9
+
+
+def _phmaxwell_flux(q, c0, chi, gph):
+    """Perfectly hyperbolic Maxwell fluxes (Munz et al. 2000) for
+    (Ex, Ey, Ez, Bx, By, Bz, phi, psi): dE/dt - c0^2 curl B + chi c0^2 grad
+    phi = 0, dB/dt + curl E + gamma grad psi = 0, dphi/dt + chi div E = 0,
+    dpsi/dt + gamma c0^2 div B = 0."""
+    Ex, Ey, Ez, Bx, By, Bz, phi, psi = q
+    c2 = c0 * c0
+    F = np.array([chi * c2 * phi, c2 * Bz, -c2 * By, gph * psi, -Ez, Ey, chi * Ex, gph * c2 * Bx])
+    G = np.array([-c2 * Bz, chi * c2 * phi, c2 * Bx, Ez, gph * psi, -Ex, chi * Ey, gph * c2 * By])
+    return F, G
+
+
+def test_phmaxwell_flux_and_lf_wave_speed():
+    """phMaxwell fluxes against the PDE, and LF with lam = c0 max(chi, gamma,
+    1) (wxphmaxwelleqn.cc:500-522): cases with chi > 1, gamma > 1 and both
+    below 1 separate it from c0, c0 max(chi, gamma) and c0 (chi + gamma)."""
+    rng = np.random.default_rng(22)
+    for c0, chi, gph in ((1.0, 0.0, 0.0), (1.3, 1.7, 0.4), (0.8, 0.3, 2.2), (2.0, 0.5, 0.25),
+                         (1.1, 1.5, 1.5)):
+        ph = O.phys(O.ORC_PHMAXWELL, c0=c0, chi=chi, gph=gph)
+        lam = c0 * max(chi, gph, 1.0)
+        for _ in range(10):
+            a, b = rng.standard_normal(8), rng.standard_normal(8)
+            th = rng.random() * 2 * np.pi
+            n = np.array([math.cos(th), math.sin(th)])
+            Fa, Ga = _phmaxwell_flux(a, c0, chi, gph)
+            Fb, Gb = _phmaxwell_flux(b, c0, chi, gph)
+            assert np.abs(O.flux(ph, 0, a) - Fa).max() <= 1e-14 * np.abs(Fa).max()
+            assert np.abs(O.flux(ph, 1, a) - Ga).max() <= 1e-14 * np.abs(Ga).max()
+            want = 0.5 * (n[0] * (Fa + Fb) + n[1] * (Ga + Gb) + lam * (a - b))
+            got, lam_o = O.numflux(ph, n, a, b)
+            assert lam_o == lam
+            assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max()
+
+
+def test_tm_upwind_flux_is_the_exact_riemann_flux():
+    """The TM upwind flux (build extension) against the exact Riemann
+    (Godunov) flux of the linear TM system along n, F* = A_n (q- + q+)/2 +
+    |A_n| (q- - q+)/2, with A_n = n_x A_x + n_y A_y written from the PDE
+    (dEz/dt = c0^2 (dBy/dx - dBx/dy), dBx/dt = -dEz/dy, dBy/dt = dEz/dx)
+    and |A_n| = R |Lambda| R^-1 from numpy's eigendecomposition.  A flipped
+    sign of the normal-B correction or dissipation on B.n fails this."""
+    rng = np.random.default_rng(23)
+    for c0 in (1.0, 1.7, 0.6):
+        ph = O.phys(O.ORC_PHMAXWELL, flux=O.ORC_FLUX_UPWIND, c0=c0)
+        c2 = c0 * c0
+        Ax = np.array([[0, 0, -c2], [0, 0, 0], [-1, 0, 0]], dtype=float)   # (Ez, Bx, By)
+        Ay = np.array([[0, c2, 0], [1, 0, 0], [0, 0, 0]], dtype=float)
+        for _ in range(20):
+            th = rng.random() * 2 * np.pi
+            n = np.array([math.cos(th), math.sin(th)])
+            An = n[0] * Ax + n[1] * Ay
+            lamv, R = np.linalg.eig(An)
+            absA = np.real(R @ np.diag(np.abs(lamv)) @ np.linalg.inv(R))
+            assert np.allclose(sorted(np.real(lamv)), [-c0, 0.0, c0], atol=1e-14)
+            a, b = np.zeros(8), np.zeros(8)
+            a[[2, 3, 4]] = rng.standard_normal(3)
+            b[[2, 3, 4]] = rng.standard_normal(3)
+            want = 0.5 * An @ (a[[2, 3, 4]] + b[[2, 3, 4]]) + 0.5 * absA @ (a[[2, 3, 4]] - b[[2, 3, 4]])
+            got, lam = O.numflux(ph, n, a, b)
+            assert lam == c0
+            assert np.abs(got[[2, 3, 4]] - want).max() <= 1e-13 * max(1.0, np.abs(want).max())
+            assert np.all(got[[0, 1, 5, 6, 7]] == 0.0)

@@ -324,3 +324,42 @@ def test_ssp_with_limiter_conserves_mass_periodic():
     m0 = np.sum(J * _ave_of(ave, q0)[:, 0])
     m1 = np.sum(J * _ave_of(ave, q1)[:, 0])
     assert len(dts) >= 4 and abs(m1 - m0) <= 1e-13 * abs(m0)
+
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4])
+def test_limiter_weights_flatten_next_to_a_zero_gradient(N):
+    """The limited gradient is sum_i w_i grad_i with w_1 = (g_2 g_3 + eps) /
+    (g_1^2 + g_2^2 + g_3^2 + 3 eps), g_i = |grad_i|^2 (Hesthaven & Warburton
+    2008, the EulerLimiter2D weights; wxhesthavenwarburtoneulerlimiter.cc:
+    317-360): when one of the three gradients is exactly zero, the other
+    two weights are eps / (g_j^2 + g_k^2) ~ 1e-13, so the element is
+    flattened to its cell average.  Set-up: zero-gradient (copy) walls and
+    a density that varies in y only, with velocity 0 and pressure 1.  On an
+    element whose face 0 lies on the wall y = 0, the ghost cell repeats the
+    cell average and both face ends carry rho(0) = 1, so that face's edge
+    gradient is exactly zero for every variable (face 0 is visited first,
+    before any state pointer is moved to a neighbour).  Weights that do not
+    multiply the other two gradients (equal weights, w_i ~ g_i, a swapped
+    index) leave a nonzero slope there; interior elements keep a slope."""
+    g = 1.4
+    mesh = E.structured_mesh(12, 0.0, 1.0, 0.0, 1.0, 0.1, 7, False)
+    ops = O.Ops(N)
+    omesh = O.Mesh(ops, mesh.VX, mesh.VY, mesh.EToV, mesh.vert_id, mesh.face_tag)
+    ph = O.phys(O.ORC_EULER, bcs=[(O.ORC_BC_COPY, [])], limiter=(g, 1e-6, 1e-6))
+    x, y = omesh.x, omesh.y
+    rho = 1.0 + 0.5 * y
+    z = np.zeros_like(rho)
+    q = np.stack([rho, z, z, z, np.full_like(rho, 1.0 / (g - 1))], axis=-1)
+    ql = O.limit(omesh, ph, q)
+    avg = np.einsum("n,knc->kc", O.limiter_weights(ops), q)
+    et = omesh.etetf
+    v = mesh.EToV
+    bottom = (et[:, 1] < 0) & (mesh.VY[v[:, 0]] == 0.0) & (mesh.VY[v[:, 1]] == 0.0)
+    assert bottom.sum() == 12
+    flat = np.abs(ql[bottom] - avg[bottom][:, None, :]).max()
+    print("N=%d: %d elements on y = 0, max |limited - cell average| %.2e" % (N, bottom.sum(), flat))
+    assert flat <= 1e-13
+    interior = ~np.any(et[:, 1::2] < 0, axis=1)
+    spread = ql[interior, :, 0].max(axis=1) - ql[interior, :, 0].min(axis=1)
+    assert spread.min() > 1e-3
