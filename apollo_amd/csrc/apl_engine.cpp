@@ -410,6 +410,15 @@ static void free_handle(apl_handle *h) {
 
 static size_t state_len(const apl_handle *h) { return (size_t)h->K * h->ops.Np * h->M; }
 
+// Whether two state-sized device ranges overlap.  The RHS and adjoint kernels
+// gather neighbour face nodes from their input while other waves store their
+// output, so an output that overlaps an input gives wrong results silently.
+static bool states_overlap(const apl_handle *h, const void *a, const void *b) {
+  const size_t bytes = sizeof(double) * state_len(h);
+  const char *pa = (const char *)a, *pb = (const char *)b;
+  return pa < pb + bytes && pb < pa + bytes;
+}
+
 static int ensure_work(apl_handle *h) {
   const size_t bytes = sizeof(double) * state_len(h);
   int rc;
@@ -458,6 +467,19 @@ int apl_mesh_structured(int n, double x0, double x1, double y0, double y1, doubl
   return APL_OK;
 }
 
+// The integer arrays of a mesh: EToV entries in [0, Nv) and face tags >= 0.
+// A negative tag would be stored as a positive "neighbour" (|tag|, |tag|) in
+// ETETF (FacePair2d writes -tag on boundary faces) and read as an interior
+// face.  Returns the reason, or NULL when the arrays are valid.
+static const char *bad_mesh_ids(const apl_mesh *mesh) {
+  for (long i = 0; i < 3L * mesh->K; ++i)
+    if (mesh->EToV[i] < 0 || mesh->EToV[i] >= mesh->Nv) return "EToV entry out of range";
+  if (mesh->face_tag)
+    for (long i = 0; i < 3L * mesh->K; ++i)
+      if (mesh->face_tag[i] < 0) return "negative face tag";
+  return nullptr;
+}
+
 int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out) {
   g_create_error.clear();
   if (!mesh || !params || !out) return fail(nullptr, APL_ERR_ARG, "null argument");
@@ -498,9 +520,7 @@ int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out)
   }
   if (mesh->K < 1 || mesh->Nv < 3 || !mesh->VX || !mesh->VY || !mesh->EToV)
     return fail(nullptr, APL_ERR_ARG, "empty or incomplete mesh");
-  for (long i = 0; i < 3L * mesh->K; ++i)
-    if (mesh->EToV[i] < 0 || mesh->EToV[i] >= mesh->Nv)
-      return fail(nullptr, APL_ERR_ARG, "EToV entry out of range");
+  if (const char *why = bad_mesh_ids(mesh)) return fail(nullptr, APL_ERR_ARG, why);
 
   apl_handle *h = new apl_handle();
   h->params = *params;
@@ -762,6 +782,7 @@ int apl_get_info(const apl_handle *h, apl_info *o) {
   o->dtscale = h->geo.dtscale;
   o->rmin = h->ops.rmin;
   o->device_bytes = h->dev_bytes;
+  o->device = h->device;
   return APL_OK;
 }
 
@@ -825,8 +846,7 @@ int apl_host_operator(int order, int which, double *out, int capacity) {
 
 int apl_host_connectivity(const apl_mesh *mesh, int *etetf) {
   if (!mesh || !etetf || mesh->K < 1 || !mesh->EToV) return APL_ERR_ARG;
-  for (long i = 0; i < 3L * mesh->K; ++i)
-    if (mesh->EToV[i] < 0 || mesh->EToV[i] >= mesh->Nv) return APL_ERR_ARG;
+  if (bad_mesh_ids(mesh)) return APL_ERR_ARG;
   std::vector<int> et;
   const int rc = face_pair(mesh->K, mesh->Nv, mesh->EToV, mesh->vert_id, mesh->face_tag, et);
   std::memcpy(etetf, et.data(), sizeof(int) * et.size());
@@ -836,6 +856,8 @@ int apl_host_connectivity(const apl_mesh *mesh, int *etetf) {
 int apl_rhs(apl_handle *h, double t, const double *q, double *rhs, double cfl,
             double *suggested_dt, int where) {
   if (!h || !q || !rhs) return APL_ERR_ARG;
+  if (where == APL_DEVICE && states_overlap(h, q, rhs))
+    return fail(h, APL_ERR_ARG, "rhs: the output overlaps the input state on the device");
   const size_t bytes = sizeof(double) * state_len(h);
   int rc;
   const double *dq = q;
@@ -1232,6 +1254,8 @@ int apl_adjoint_rhs(apl_handle *h, double t, const double *q, const double *lam,
                     int where) {
   if (!h || !q || !lam || !out) return APL_ERR_ARG;
   if (h->params.limiter) return fail(h, APL_ERR_UNSUPPORTED, "the adjoint does not cover the limiter");
+  if (where == APL_DEVICE && (states_overlap(h, q, out) || states_overlap(h, lam, out)))
+    return fail(h, APL_ERR_ARG, "adjoint_rhs: the output overlaps an input on the device");
   int rc;
   if ((rc = ensure_adjoint_ops(h)) || (rc = ensure_work(h))) return rc;
   const size_t bytes = sizeof(double) * state_len(h);

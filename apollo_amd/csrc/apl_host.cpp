@@ -181,7 +181,7 @@ bool build_operators(int N, Operators &o) {
   o.Nfp = N + 1;
   o.Ng = 2 * (N + 1);                            // wxcubature2d.cc:23
   const int cub_order = 3 * (N + 1);             // wxcubature2d.cc:20
-  o.Nc = apl_tab_cub_npts[cub_order - 1];        // wxcubature2d.cc:26-28
+  o.Nc = apl_tab_cub_npts[N - 1];                // wxcubature2d.cc:26-28 (rule cub_order)
   const int Np = o.Np, Nc = o.Nc, Ng = o.Ng;
   o.r.assign(apl_tab_node_r + apl_tab_node_off[N - 1], apl_tab_node_r + apl_tab_node_off[N]);
   o.s.assign(apl_tab_node_s + apl_tab_node_off[N - 1], apl_tab_node_s + apl_tab_node_off[N]);
@@ -203,7 +203,7 @@ bool build_operators(int N, Operators &o) {
   o.cr.resize(Nc);
   o.cs.resize(Nc);
   o.cw.resize(Nc);
-  const double *rule = apl_tab_cub_rsw + apl_tab_cub_off[cub_order - 1];
+  const double *rule = apl_tab_cub_rsw + apl_tab_cub_off[N - 1];   // cub2D_<cub_order>
   for (int k = 0; k < Nc; ++k) {
     o.cr[k] = rule[3 * k];
     o.cs[k] = rule[3 * k + 1];

@@ -62,7 +62,7 @@ class _Info(ctypes.Structure):
     _fields_ = [("K", ctypes.c_int), ("N", ctypes.c_int), ("Np", ctypes.c_int),
                 ("Nc", ctypes.c_int), ("Ng", ctypes.c_int), ("Nfp", ctypes.c_int),
                 ("m", ctypes.c_int), ("dtscale", ctypes.c_double), ("rmin", ctypes.c_double),
-                ("device_bytes", ctypes.c_longlong)]
+                ("device_bytes", ctypes.c_longlong), ("device", ctypes.c_int)]
 
 
 EXPORTS = [
@@ -262,6 +262,7 @@ class Engine:
             info.K, info.N, info.Np, info.Nc, info.Ng, info.Nfp, info.m)
         self.dtscale, self.rmin = info.dtscale, info.rmin
         self.device_bytes = info.device_bytes
+        self.device = info.device
 
     def close(self):
         if getattr(self, "_h", None):
@@ -306,32 +307,55 @@ class Engine:
         return out
 
     # -- compute -------------------------------------------------------------
-    @staticmethod
-    def _ptr(a):
-        """(pointer, where) of a numpy array or a CUDA/HIP torch tensor."""
+    def _ptr(self, a, name="state"):
+        """(pointer, where) of a state-sized numpy array or torch tensor.
+
+        The C-ABI reads and writes K*Np*m doubles at the pointer, so anything
+        else is refused here: a dtype other than float64, another element
+        count, a non-contiguous layout, or a device tensor on a GPU other
+        than the handle's."""
+        n = self.K * self.Np * self.m
         if isinstance(a, np.ndarray):
-            assert a.dtype == np.float64 and a.flags.c_contiguous
+            if a.dtype != np.float64 or not a.flags.c_contiguous:
+                raise ValueError("%s: need a C-contiguous float64 array, got %s" % (name, a.dtype))
+            if a.size != n:
+                raise ValueError("%s: need K*Np*m = %d values, got %d" % (name, n, a.size))
             return a.ctypes.data, HOST
-        assert a.dtype.is_floating_point and a.is_contiguous()
-        return a.data_ptr(), (DEVICE if a.is_cuda else HOST)
+        if str(getattr(a, "dtype", "")) != "torch.float64" or not a.is_contiguous():
+            raise ValueError("%s: need a contiguous torch.float64 tensor, got %s"
+                             % (name, getattr(a, "dtype", type(a))))
+        if a.numel() != n:
+            raise ValueError("%s: need K*Np*m = %d values, got %d" % (name, n, a.numel()))
+        if a.is_cuda:
+            if a.device.index != self.device:
+                raise ValueError("%s: tensor on cuda:%d, handle on device %d"
+                                 % (name, a.device.index, self.device))
+            return a.data_ptr(), DEVICE
+        return a.data_ptr(), HOST
+
+    def _pair(self, a, b, na, nb):
+        """Pointers of two state arguments that must live on the same side."""
+        pa, wa = self._ptr(a, na)
+        pb, wb = self._ptr(b, nb)
+        if wa != wb:
+            raise ValueError("%s and %s must both be host or both be device memory" % (na, nb))
+        return pa, pb, wa
 
     def rhs(self, t, q, out, cfl=1.0):
         """Apollo step(): out = RHS(q).  Returns the suggested dt."""
-        qp, where = self._ptr(q)
-        op, w2 = self._ptr(out)
-        assert where == w2
+        qp, op, where = self._pair(q, out, "q", "out")
         dt = ctypes.c_double(0.0)
         self._check(lib().apl_rhs(self._h, t, qp, op, cfl, ctypes.byref(dt), where), "rhs")
         return dt.value
 
     def initial_dt(self, q, cfl):
-        qp, where = self._ptr(q)
+        qp, where = self._ptr(q, "q")
         dt = ctypes.c_double(0.0)
         self._check(lib().apl_initial_dt(self._h, qp, cfl, ctypes.byref(dt), where), "initial_dt")
         return dt.value
 
     def lserk4(self, t0, dt, nsteps, q):
-        qp, where = self._ptr(q)
+        qp, where = self._ptr(q, "q")
         self._check(lib().apl_lserk4(self._h, t0, dt, nsteps, qp, where), "lserk4")
 
     def ssp_advance(self, t0, t_end, dt0, cfl, q, stages=5, max_steps=1000000):
@@ -347,28 +371,24 @@ class Engine:
     def limit(self, q_in, q_out):
         """Apollo's Hesthaven-Warburton Euler limiter: q_out = limited q_in
         (q_out may be q_in)."""
-        ip, where = self._ptr(q_in)
-        op, w2 = self._ptr(q_out)
-        assert where == w2
+        ip, op, where = self._pair(q_in, q_out, "q_in", "q_out")
         self._check(lib().apl_limit(self._h, ip, op, where), "limit")
 
     def adjoint_rhs(self, t, q, lam, out):
-        qp, where = self._ptr(q)
-        lp, _ = self._ptr(lam)
-        op, _ = self._ptr(out)
+        qp, lp, where = self._pair(q, lam, "q", "lam")
+        op, _, _ = self._pair(out, q, "out", "q")
         self._check(lib().apl_adjoint_rhs(self._h, t, qp, lp, op, where), "adjoint_rhs")
 
     def lserk4_adjoint(self, t0, dt, nsteps, ckpt_every, q0, qbarT, dJdq0, qT=None):
-        q0p, where = self._ptr(q0)
-        qbp, _ = self._ptr(qbarT)
-        gp, _ = self._ptr(dJdq0)
-        qtp = self._ptr(qT)[0] if qT is not None else None
+        q0p, qbp, where = self._pair(q0, qbarT, "q0", "qbarT")
+        gp, _, _ = self._pair(dJdq0, q0, "dJdq0", "q0")
+        qtp = self._pair(qT, q0, "qT", "q0")[0] if qT is not None else None
         self._check(lib().apl_lserk4_adjoint(self._h, t0, dt, nsteps, ckpt_every, q0p, qbp, gp,
                                              qtp, where), "lserk4_adjoint")
 
     def objective_disc_sq(self, q, var, xc, yc, radius, dJdq=None):
-        qp, where = self._ptr(q)
-        gp = self._ptr(dJdq)[0] if dJdq is not None else None
+        qp, where = self._ptr(q, "q")
+        gp = self._pair(dJdq, q, "dJdq", "q")[0] if dJdq is not None else None
         J = ctypes.c_double(0.0)
         self._check(lib().apl_objective_disc_sq(self._h, qp, var, xc, yc, radius, ctypes.byref(J),
                                                 gp, where), "objective")

@@ -131,7 +131,8 @@ typedef struct {
                             (identity).  Periodic meshes map each image vertex
                             to the id of its partner.                          */
   const int *face_tag;   /* [K*3] Gmsh physical tag of face f of element k
-                            ("Face Sets" label), 0 = none; NULL = no tags      */
+                            ("Face Sets" label), >= 0, 0 = none; NULL = no
+                            tags.  A negative tag is APL_ERR_ARG.             */
 } apl_mesh;
 
 typedef struct {
@@ -162,6 +163,7 @@ typedef struct {
   double dtscale;        /* min inscribed radius (wxnodaldggeometry2d.cc:220-227) */
   double rmin;           /* |r1 - r0|          (wxnodaldggeometry2d.cc:91)         */
   long long device_bytes;/* device memory held by the handle                     */
+  int device;            /* HIP device the handle was created on                 */
 } apl_info;
 
 typedef struct apl_handle_s apl_handle;
@@ -210,7 +212,9 @@ int apl_get_operator(const apl_handle *h, int which, double *out, int capacity);
 /* Replaces WxNodalDG2dMethod<double>::step (wxnodaldg2dmethod.cc:235-542):
  * rhs = M^-1 [ volume - surface ] / J for every element, overwriting `rhs`.
  * `suggested_dt` (may be NULL) receives (2/3) cfl dtscale rmin / maxSpeed
- * (wxnodaldg2dmethod.cc:523) with `cfl` as given. */
+ * (wxnodaldg2dmethod.cc:523) with `cfl` as given.  With APL_DEVICE, `rhs`
+ * must not overlap `q` (APL_ERR_ARG): the kernels gather neighbour nodes of
+ * q while other elements' results are stored. */
 int apl_rhs(apl_handle *h, double t, const double *q, double *rhs,
             double cfl, double *suggested_dt, int where);
 
@@ -254,7 +258,8 @@ int apl_ssp_advance(apl_handle *h, double t0, double t_end, double dt0, double c
 
 /* ---- discrete adjoint (no counterpart in the reference) ----------------- */
 
-/* out = (d rhs / d q)^T lam at state q and time t (FP64). */
+/* out = (d rhs / d q)^T lam at state q and time t (FP64).  With APL_DEVICE,
+ * `out` must not overlap `q` or `lam` (APL_ERR_ARG). */
 int apl_adjoint_rhs(apl_handle *h, double t, const double *q, const double *lam,
                     double *out, int where);
 

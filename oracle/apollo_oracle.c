@@ -218,7 +218,7 @@ int orc_build_ops(int N, orc_ops *o) {
   int Np = (N + 1) * (N + 2) / 2, Nfp = N + 1;
   int cubOrd = (int)floor(3.0 * (N + 1));          /* wxcubature2d.cc:20 */
   int Ng = (int)floor((N + 1) * 2);                 /* wxcubature2d.cc:23 */
-  int Nc = apl_tab_cub_npts[cubOrd - 1];            /* wxcubature2d.cc:26-28 */
+  int Nc = apl_tab_cub_npts[N - 1];                 /* wxcubature2d.cc:26-28, rule cubOrd */
   o->N = N; o->Np = Np; o->Nfp = Nfp; o->Nc = Nc; o->Ng = Ng;
   o->r = ALLOC(double, Np); o->s = ALLOC(double, Np);
   o->fmask = ALLOC(int, 3 * Nfp);
@@ -247,7 +247,7 @@ int orc_build_ops(int N, orc_ops *o) {
   /* cubature (wxcubature2d.cc:29-75) */
   o->cr = ALLOC(double, Nc); o->cs = ALLOC(double, Nc); o->cw = ALLOC(double, Nc);
   for (int k = 0; k < Nc; k++) {
-    const double *t = apl_tab_cub_rsw + apl_tab_cub_off[cubOrd - 1] + 3 * k;
+    const double *t = apl_tab_cub_rsw + apl_tab_cub_off[N - 1] + 3 * k;   /* cub2D_<cubOrd> */
     o->cr[k] = t[0]; o->cs[k] = t[1]; o->cw[k] = t[2];
   }
   o->Vc = ALLOC(double, Nc * Np); o->Drc = ALLOC(double, Nc * Np); o->Dsc = ALLOC(double, Nc * Np);

@@ -90,6 +90,41 @@ def test_host_connectivity_matches_oracle(periodic):
         assert (a < 0).sum() == 2 * 4 * 9     # 36 boundary faces, two entries each
 
 
+def test_negative_face_tag_is_rejected():
+    """FacePair2d stores -tag on boundary faces, so a negative tag would read
+    as an interior neighbour (|tag|, |tag|); both entry points refuse it."""
+    mesh = E.structured_mesh(3, 0, 1, 0, 1)
+    tags = mesh.face_tag.copy()
+    tags.flat[np.argmax(tags.ravel() > 0)] = -2
+    bad = E.MeshData(mesh.VX, mesh.VY, mesh.EToV, None, tags)
+    with pytest.raises(E.ApolloError) as ei:
+        E.host_connectivity(bad)
+    assert ei.value.status == 1
+    with pytest.raises(E.ApolloError) as ei:
+        E.Engine(bad, E.Params(3, E.EQN_MAXWELL_TM3, bcs=[(E.BC_PEC, [])] * 2))
+    assert ei.value.status == 1
+
+
+def test_state_arguments_are_checked():
+    """The wrappers refuse a state the C-ABI would read past or misread:
+    wrong dtype, wrong size, non-contiguous, or host/device mixed."""
+    import types
+    fake = types.SimpleNamespace(K=4, Np=3, m=2, device=0)
+    ptr = lambda a, name="q": E.Engine._ptr(fake, a, name)
+    good = np.zeros((4, 3, 2))
+    assert ptr(good)[1] == E.HOST
+    for bad in (np.zeros((4, 3, 2), dtype=np.float32), np.zeros((4, 3, 3)),
+                np.zeros((4, 6, 2))[:, ::2], [0.0] * 24):
+        with pytest.raises(ValueError):
+            ptr(bad)
+    torch = pytest.importorskip("torch")
+    assert ptr(torch.zeros(24, dtype=torch.float64))[1] == E.HOST
+    for bad in (torch.zeros(24, dtype=torch.float32), torch.zeros(25, dtype=torch.float64),
+                torch.zeros(48, dtype=torch.float64)[::2]):
+        with pytest.raises(ValueError):
+            ptr(bad)
+
+
 @pytest.mark.parametrize("name", ["vortex_64_N3", "tm_cavity_256_N5", "vortex_512_N4",
                                   "adjoint_vortex_256_N3", "tm_fourier_384_N8"])
 def test_workload_meshes_full_size(name):
