@@ -170,8 +170,9 @@ def run_leg(name, variants, torch, steps=None):
 def run_adjoint(torch):
     """Reverse LSERK4 on adjoint_vortex_256_N3: timing, and dJ/dq0 against
     central finite differences along 8 Gaussian directions of two kinds,
-    smooth random fields and white noise, at eps = 1e-6 (and white noise at
-    1e-7 as well, to show the eps-scaling of its error)."""
+    smooth random fields and white noise, at eps = 1e-6 (white noise also at
+    1e-5 and 1e-7, to show how its error scales with eps), with the rounding
+    noise of J that limits the white-noise differences."""
     from apollo_amd import workloads as W
     w = W.get("adjoint_vortex_256_N3")
     steps, ckpt = w.steps, w.ckpt
@@ -217,8 +218,16 @@ def run_adjoint(torch):
             ad = float((grad * d).sum())
             return abs(fd - ad) / abs(ad)
 
+        # rounding noise of J: its change when q0 moves by -1, 0 or +1 ulp per entry
+        noise = 0.0
+        for sd in (1, 2, 3):
+            r = np.random.default_rng(sd).integers(-1, 2, size=q0.shape).astype(np.float64)
+            noise = max(noise, abs(objective(q0d * (1.0 + np.finfo(np.float64).eps *
+                                                    torch.from_numpy(r).cuda())) - J))
         gen = torch.Generator(device="cpu").manual_seed(8)
         white = [torch.randn(q0d.shape, generator=gen, dtype=torch.float64).cuda() for _ in range(8)]
+        gd_white = [abs(float((grad * d).sum())) for d in white]
+        errs5 = [fd_err(d, 1e-5) for d in white]
         errs6 = [fd_err(d, 1e-6) for d in white]
         errs7 = [fd_err(d, 1e-7) for d in white]
         # smooth Gaussian directions: 16 random Fourier modes of the periodic
@@ -242,13 +251,16 @@ def run_adjoint(torch):
                fd_directions=8, fd_eps=1e-6,
                fd_max_rel_err_smooth=max(errs_smooth),
                fd_max_rel_err_white_noise=max(errs6),
+               fd_max_rel_err_white_noise_eps1e5=max(errs5),
                fd_max_rel_err_white_noise_eps1e7=max(errs7),
+               J_rounding_noise=noise,
+               fd_white_noise_predicted_rounding_err_eps1e6=max(noise / (1e-6 * g) for g in gd_white),
                fd_note="smooth: Gaussian random fields (16 Fourier modes per variable, N(0,1) "
-                       "amplitudes); white noise: N(0,1) per degree of freedom.  The LF speed "
-                       "max(|u-|+c-, |u+|+c+) has a kink where the traces' speeds cross; white noise "
-                       "moves the two traces of a face independently by ~eps, so a fraction ~eps of "
-                       "the face points cross it and the central-difference error falls like eps "
-                       "(compare eps 1e-6 and 1e-7); smooth directions move both traces together",
+                       "amplitudes); white noise: N(0,1) per degree of freedom.  Along white noise "
+                       "g.d sums 5.2M terms of random sign and is small next to J, so the central "
+                       "difference is limited by the rounding noise of J: its error is about "
+                       "noise(J) / (eps |g.d|) (the predicted figure) and grows as eps shrinks "
+                       "(compare eps 1e-5, 1e-6, 1e-7)",
                note="adjoint DOF-updates/s counts the 5 x steps adjoint RHS evaluations of the "
                     "reverse sweep over the whole reverse LSERK4 call (forward sweep, segment "
                     "recompute and stage rebuild included)")

@@ -739,9 +739,21 @@ void orc_set_exact_geometry(int on) { g_exact_geom = on; }
 static void cub_geom(const orc_ops *o, const double *xcoord, const double *ycoord,
                      const double *geo, double *xr, double *xs, double *yr, double *ys);
 
+/* With reverse_sums set, every matrix-vector product sums its terms in the
+ * reverse order: another FP64 evaluation of the same algorithm, which
+ * measures the RHS's rounding floor (a diagnostic; off by default). */
+static int g_reverse_sums = 0;
+void orc_set_reverse_sums(int on) { g_reverse_sums = on; }
+
 /* MatrixVectorMult (wxcubature2d.cc:431-442). */
 static void mvm(int rows, int cols, int meqn, const double *A, const double *x, double *y) {
   for (int k = 0; k < rows * meqn; k++) y[k] = 0.0;
+  if (g_reverse_sums) {
+    for (int kx = 0; kx < rows; kx++)
+      for (int ky = cols - 1; ky >= 0; ky--)
+        for (int kz = 0; kz < meqn; kz++) y[kx * meqn + kz] += A[kx * cols + ky] * x[ky * meqn + kz];
+    return;
+  }
   for (int kx = 0; kx < rows; kx++)
     for (int ky = 0; ky < cols; ky++)
       for (int kz = 0; kz < meqn; kz++) y[kx * meqn + kz] += A[kx * cols + ky] * x[ky * meqn + kz];

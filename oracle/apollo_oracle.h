@@ -119,6 +119,8 @@ int orc_rhs(const orc_ops *ops, const orc_phys *ph, int K, const double *x,
  * cubature points instead of Apollo's Dr_c x / Ds_c x recomputation
  * (wxcubature2d.cc:305,327-356).  Off (0) by default; process-global. */
 void orc_set_exact_geometry(int on);
+/* Diagnostic: every matrix-vector product of the RHS sums in reverse order. */
+void orc_set_reverse_sums(int on);
 
 /* Initial dt (wxnodaldg2dmethod.cc:201-228) using Roe speeds of a state
  * against itself (wxeulereqn.cc:95-200) / c0 max(chi,gamma,1) for Maxwell. */

@@ -88,6 +88,7 @@ def lib():
                                          ctypes.c_int, _dp, _dp, _ip, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_int, _dp, _dp, _dp]
         L.orc_set_exact_geometry.argtypes = [ctypes.c_int]
+        L.orc_set_reverse_sums.argtypes = [ctypes.c_int]
         L.orc_objective_disc_sq.restype = ctypes.c_double
         L.orc_objective_disc_sq.argtypes = [ctypes.POINTER(_Ops), ctypes.c_int, _dp, _dp,
                                             ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double,
@@ -263,6 +264,19 @@ class exact_geometry:
 
     def __exit__(self, *exc):
         lib().orc_set_exact_geometry(0)
+        return False
+
+
+class reverse_sums:
+    """Context manager: every matrix-vector product of the oracle RHS sums
+    its terms in reverse order (see orc_set_reverse_sums)."""
+
+    def __enter__(self):
+        lib().orc_set_reverse_sums(1)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_reverse_sums(0)
         return False
 
 

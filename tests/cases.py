@@ -84,13 +84,19 @@ class Case:
         return from_oracle(self.params.eqn, r), ms
 
 
-def oracle_ulp_sensitivity(case, q, t=0.0, seeds=(1, 2)):
-    """Rounding floor of the FP64 RHS at state q: the largest relative L2
-    change of the exact-geometry oracle's RHS when every entry of q is moved
-    by -1, 0 or +1 ulp at random (one draw per seed).  Two FP64 evaluations
-    that order their operations differently cannot be expected to agree
-    more closely than this: on fine meshes the volume and surface terms, each
-    of order |F| / h, cancel to a much smaller RHS."""
+def rounding_floor(case, q, t=0.0, seeds=(1, 2)):
+    """Rounding floor of the FP64 RHS at state q, measured on the
+    exact-geometry oracle in two ways, of which the larger is returned:
+      - its 1-ulp input sensitivity: the largest relative L2 change of the
+        RHS when every entry of q is moved by -1, 0 or +1 ulp at random (one
+        draw per seed);
+      - the relative L2 difference between the oracle and the same oracle
+        with every matrix-vector product summed in reverse order.
+    Two FP64 evaluations that order their operations differently cannot be
+    expected to agree more closely than this.  On fine meshes it is well
+    above 1e-12 for smooth states: the volume and surface terms, each of
+    order |F| / h, cancel to a much smaller RHS (in the far field of the
+    vortex to almost nothing)."""
     eps = np.finfo(np.float64).eps
     with O.exact_geometry():
         ref, _ = case.oracle_rhs(q, t)
@@ -99,10 +105,12 @@ def oracle_ulp_sensitivity(case, q, t=0.0, seeds=(1, 2)):
             r = np.random.default_rng(s).integers(-1, 2, size=q.shape)
             moved, _ = case.oracle_rhs(np.ascontiguousarray(q * (1.0 + eps * r)), t)
             worst = max(worst, rel_l2(moved, ref))
-    return worst
+        with O.reverse_sums():
+            rev, _ = case.oracle_rhs(q, t)
+    return max(worst, rel_l2(rev, ref))
 
 
-def fp64_parity(case, got, q, t=0.0, tol=1e-12, ulp_floor=False):
+def fp64_parity(case, got, q, t=0.0, tol=1e-12, floor=None):
     """FP64 parity bar.  The engine evaluates the geometric factors of each
     straight-sided element exactly, Apollo recomputes them at every cubature
     point from Dr_c x (wxcubature2d.cc:305), which adds rounding noise of order
@@ -110,26 +118,22 @@ def fp64_parity(case, got, q, t=0.0, tol=1e-12, ulp_floor=False):
     exact geometric factors, and no farther from the faithful oracle than
     max(tol, 2 x that noise).
 
-    With ulp_floor, the exact-geometry bar is max(tol, 2 x the oracle's own
-    1-ulp input sensitivity, oracle_ulp_sensitivity): on the full-size
-    vortex meshes that sensitivity is 6.3e-13 (64x64, N=3), 2.5e-12
-    (256x256, N=3) and 7.4e-12 (512x512, N=4), so a flat 1e-12 would sit
-    below what any differently ordered FP64 evaluation can reach.
-    Returns (err_exact, err_faithful, noise)."""
+    With `floor` (rounding_floor of this state), the exact-geometry bar is
+    max(tol, 4 x floor): the engine differs from the oracle by its summation
+    order, its folded operators (V V^T, and the cubature folded into nodal
+    operators for linear fluxes) and its exact geometric factors, each an
+    FP64 rounding of about that size.  Returns (err_exact, err_faithful,
+    noise)."""
     ref, _ = case.oracle_rhs(q, t)
     with O.exact_geometry():
         ref_exact, _ = case.oracle_rhs(q, t)
     noise = rel_l2(ref, ref_exact)
     e_exact, e_faith = rel_l2(got, ref_exact), rel_l2(got, ref)
-    bar = tol
-    sens = None
-    if ulp_floor:
-        sens = oracle_ulp_sensitivity(case, q, t)
-        bar = max(tol, 2.0 * sens)
+    bar = tol if floor is None else max(tol, 4.0 * floor)
     print("%s: rel L2 vs exact-geometry oracle %.3e, vs faithful oracle %.3e "
           "(oracle q8 noise %.3e%s; bar %.2e)"
           % (case.name, e_exact, e_faith, noise,
-             "" if sens is None else ", oracle 1-ulp sensitivity %.3e" % sens, bar))
+             "" if floor is None else ", FP64 rounding floor %.3e" % floor, bar))
     assert e_exact <= bar, (e_exact, bar)
     assert e_faith <= max(bar, 2.0 * noise), (e_faith, noise)
     return e_exact, e_faith, noise

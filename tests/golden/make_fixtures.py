@@ -8,9 +8,10 @@ Reads reference data files as text (nothing is compiled, imported or run):
   differentiation p_Dr/p_Ds, the strong-form lift p_LIFT and NUDG's face
   masks p_Fmask.  Stored as ``nudg_N<N>.npz`` (plain arrays, no pickle).
 * five Gmsh v2.2 meshes from ``examples/unstructuredDG`` copied verbatim into
-  ``meshes/`` (data the reference ships with its example decks), and four
-  larger ones, those of the Euler shock decks, gzip-compressed (``.msh.gz``,
-  the text unchanged).
+  ``meshes/`` (data the reference ships with its example decks), and six
+  larger ones gzip-compressed (``.msh.gz``, the text unchanged): those of the
+  Euler shock decks, and those of the isentropic-vortex and
+  transverse-magnetic decks (grid2.msh, square3.msh).
 
 Usage: python tests/golden/make_fixtures.py [/root/reference]
 """
@@ -39,6 +40,8 @@ GZ_MESHES = {
     "scramjet_grid.msh.gz": "examples/unstructuredDG/euler/scramjet/grid.msh",
     "explosion_gridBox.msh.gz": "examples/unstructuredDG/euler/explosionTest/gridBox.msh",
     "bfs_grid.msh.gz": "examples/unstructuredDG/euler/backwardFacingStep/grid.msh",
+    "vortex_grid2.msh.gz": "examples/unstructuredDG/euler/isentropicVortex/grid2.msh",
+    "tm_square3.msh.gz": "examples/unstructuredDG/maxwell/transverseMagnetic/square3.msh",
 }
 
 
