@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -73,7 +74,49 @@ struct apl_handle_s {
   bool ltr_is_state = false;   // trace slots = nodes 0..Np-1 of a 5-variable state
   // pinned host words the flag and speed read-backs land in (two doubles)
   double *h_pin = nullptr;
+  // this device's copy of apl_rhs_noalloc's scratch array (not owned)
+  double *scratch = nullptr;
+  std::string scratch_error;
 };
+
+// ---------------------------------------------------------------------------
+// the device scratch of apl_rhs_noalloc
+// ---------------------------------------------------------------------------
+// apl_rhs_noalloc uses this file-scope array instead of the handle's
+// buffers, so that it allocates nothing.  Layout in doubles: [0] the reduced
+// speed maximum, [1] the error flags (its low 32 bits), [2, 32) unused,
+// [32, 32 + nb) the per-batch speed maxima of one RHS over nb = ceil(K/16)
+// batches.  Every GPU the library runs on holds its own copy (8 MB) until
+// the process ends; calls that overlap on one GPU share it and race.
+constexpr int NOALLOC_HEAD = 32;
+constexpr long NOALLOC_MAX_BATCHES = 1L << 20;            // K <= 16 777 216
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[NOALLOC_HEAD + NOALLOC_MAX_BATCHES];
+
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of this device's copy of `scratch` (the array's name is
+// a host-side stand-in, not a device address), looked up once per device
+// with that device current and kept for later handles.  A failure is kept
+// on the handle and reported by apl_rhs_noalloc; nothing else needs it.
+static void scratch_setup(apl_handle *h) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (h->device < 0 || h->device >= NOALLOC_MAX_DEVICES) {
+    h->scratch_error = "device id " + std::to_string(h->device) + " beyond the scratch table";
+    return;
+  }
+  if (!g_scratch_ptr[h->device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      h->scratch_error = std::string("hipGetSymbolAddress: ") + hipGetErrorString(e);
+      return;
+    }
+    g_scratch_ptr[h->device] = (double *)p;
+  }
+  h->scratch = g_scratch_ptr[h->device];
+}
 
 // ---------------------------------------------------------------------------
 // helpers
@@ -164,17 +207,21 @@ static RhsArgs base_args(apl_handle *h) {
   return a;
 }
 
+// Status of a set of kernel error flags.
+static int flags_status(apl_handle *h, unsigned ef) {
+  if (ef & EF_DENSITY) return fail(h, APL_ERR_NEG_DENSITY, "negative density in the Euler flux");
+  if (ef & EF_PRESSURE) return fail(h, APL_ERR_NEG_PRESSURE, "negative pressure in the Euler flux");
+  if (ef & EF_NAN) return fail(h, APL_ERR_NAN, "NaN in the RHS");
+  return APL_OK;
+}
+
 // Read the error flags after everything enqueued so far (one stream
 // synchronisation).
 static int check_flags(apl_handle *h) {
   unsigned *pf = (unsigned *)h->h_pin;
   HIPCHK(h, hipMemcpyAsync(pf, h->d_err, sizeof(unsigned), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  const unsigned ef = *pf;
-  if (ef & EF_DENSITY) return fail(h, APL_ERR_NEG_DENSITY, "negative density in the Euler flux");
-  if (ef & EF_PRESSURE) return fail(h, APL_ERR_NEG_PRESSURE, "negative pressure in the Euler flux");
-  if (ef & EF_NAN) return fail(h, APL_ERR_NAN, "NaN in the RHS");
-  return APL_OK;
+  return flags_status(h, *pf);
 }
 
 // Fold and permute the operators into MFMA fragments (see apl_device.h).
@@ -589,6 +636,7 @@ int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out)
     return bail(APL_ERR_HIP);
   }
   h->own_stream = true;
+  scratch_setup(h);
   std::vector<double> g((size_t)h->K * G_WORDS, 0.0);
   for (int k = 0; k < h->K; ++k) {
     const double *s = &h->geo.geom[(size_t)14 * k];
@@ -888,6 +936,43 @@ int apl_rhs(apl_handle *h, double t, const double *q, double *rhs, double cfl,
     // wxnodaldg2dmethod.cc:523
     *suggested_dt = 2. / 3. * cfl * h->geo.dtscale * (h->ops.rmin / ms);
   }
+  return APL_OK;
+}
+
+int apl_rhs_noalloc(apl_handle *h, double t, const double *q, double *rhs, double cfl,
+                    double *suggested_dt) {
+  if (!h || !q || !rhs) return APL_ERR_ARG;
+  if (!h->scratch)
+    return fail(h, APL_ERR_HIP, "rhs_noalloc: no device scratch array (" + h->scratch_error + ")");
+  const long nb = ((long)h->K + 15) / 16;
+  if (nb > NOALLOC_MAX_BATCHES)
+    return fail(h, APL_ERR_ARG, "rhs_noalloc: K = " + std::to_string(h->K) +
+                                    " exceeds the scratch capacity of " +
+                                    std::to_string(16 * NOALLOC_MAX_BATCHES) + " elements");
+  if (states_overlap(h, q, rhs))
+    return fail(h, APL_ERR_ARG, "rhs_noalloc: the output overlaps the input state");
+  // the flags and the reduced maximum start at zero (the array keeps the
+  // last call's contents)
+  HIPCHK(h, hipMemsetAsync(h->scratch, 0, sizeof(double) * NOALLOC_HEAD, h->stream));
+  RhsArgs a = base_args(h);
+  a.mode = 0;
+  a.q = q;
+  a.out = rhs;
+  a.t = t;
+  a.err = (unsigned *)(h->scratch + 1);
+  a.speed_part = h->scratch + NOALLOC_HEAD;
+  int rc;
+  if ((rc = dispatch_rhs(h, a))) return rc;
+  if (suggested_dt && launch_max_reduce(h->stream, h->scratch + NOALLOC_HEAD, (int)nb, h->scratch))
+    return fail(h, APL_ERR_HIP, "rhs_noalloc: max kernel");
+  double back[2];
+  HIPCHK(h, hipMemcpyAsync(back, h->scratch, sizeof(back), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  unsigned ef;
+  std::memcpy(&ef, &back[1], sizeof(ef));
+  if ((rc = flags_status(h, ef))) return rc;
+  // wxnodaldg2dmethod.cc:523
+  if (suggested_dt) *suggested_dt = 2. / 3. * cfl * h->geo.dtscale * (h->ops.rmin / back[0]);
   return APL_OK;
 }
 

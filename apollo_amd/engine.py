@@ -69,7 +69,7 @@ EXPORTS = [
     "apl_create", "apl_destroy", "apl_get_info", "apl_set_stream", "apl_get_index_maps",
     "apl_get_geometry", "apl_get_operator", "apl_rhs", "apl_initial_dt", "apl_lserk4",
     "apl_ssp_advance", "apl_limit", "apl_adjoint_rhs", "apl_lserk4_adjoint", "apl_objective_disc_sq",
-    "apl_mesh_structured",
+    "apl_mesh_structured", "apl_rhs_noalloc",
     "apl_status_string", "apl_last_error", "apl_host_connectivity", "apl_host_operator",
 ]
 
@@ -101,6 +101,7 @@ def lib():
     L.apl_get_geometry.argtypes = [vp, _dp, _dp, _dp]
     L.apl_get_operator.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_int]
     L.apl_rhs.argtypes = [vp, ctypes.c_double, vp, vp, ctypes.c_double, _dp, ctypes.c_int]
+    L.apl_rhs_noalloc.argtypes = [vp, ctypes.c_double, vp, vp, ctypes.c_double, _dp]
     L.apl_initial_dt.argtypes = [vp, vp, ctypes.c_double, _dp, ctypes.c_int]
     L.apl_lserk4.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, ctypes.c_int]
     L.apl_ssp_advance.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,

@@ -218,6 +218,19 @@ int apl_get_operator(const apl_handle *h, int which, double *out, int capacity);
 int apl_rhs(apl_handle *h, double t, const double *q, double *rhs,
             double cfl, double *suggested_dt, int where);
 
+/* apl_rhs on device pointers that allocates nothing and calls no
+ * allocating routine, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same kernels and results
+ * as apl_rhs(..., APL_DEVICE).  Its per-batch speed maxima and error flags
+ * live in a static device array of the library (8 MB per GPU, held until the
+ * process ends) instead of the handle's buffers; K above 16 777 216 elements
+ * is APL_ERR_ARG.  Calls that overlap on one GPU share that array and race:
+ * the caller serialises them.  Synchronises the handle's stream once to read
+ * the flags and the speed maximum back. */
+int apl_rhs_noalloc(apl_handle *h, double t, const double *q, double *rhs, double cfl,
+                    double *suggested_dt);
+
 /* Replaces WxNodalDG2dMethod::applyLimiter -> WxHestavenWarburtonEulerLimiter::
  * applyLimiter (wxnodaldg2dmethod.cc:575-582, src/hyperapps/euler/
  * wxhesthavenwarburtoneulerlimiter.cc:61-484): every element is replaced
