@@ -43,6 +43,11 @@ enum apl_status {
   APL_ERR_ALLOC = 10,       /* host or device allocation failed               */
   APL_ERR_UNSUPPORTED = 11  /* valid request this build does not provide      */
 };
+/* The kernels raise flags and the host reads them once per call (once per
+ * step in the steppers).  When one call raises several, the status is the
+ * first of NEG_DENSITY, NEG_PRESSURE, NAN.  Apollo exits at the first failing
+ * node instead, so where a state has both a negative density and a
+ * negative pressure the two may name different ones. */
 
 /* ---- equation systems --------------------------------------------------- */
 enum apl_eqn {
