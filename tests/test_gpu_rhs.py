@@ -239,6 +239,46 @@ def test_lserk4_graph_matches_oracle():
     assert rel_l2(q, ref) <= 1e-13
 
 
+@pytest.mark.parametrize("N,n,flux,steps", [(1, 41, E.FLUX_LF, 3), (2, 41, E.FLUX_UPWIND, 3),
+                                             (3, 41, E.FLUX_UPWIND, 3), (4, 41, E.FLUX_LF, 3),
+                                             (5, 41, E.FLUX_UPWIND, 3), (5, 161, E.FLUX_UPWIND, 2)])
+def test_nodal_lserk4_matches_oracle(N, n, flux, steps):
+    """LSERK4 steps of the nodal form (Maxwell TM, PEC walls) against the
+    oracle's LSERK4:
+    K = 2 n^2 = 3362 and 51842, both with a partial last batch (2 elements),
+    the larger with several batches per wave.  FP64 increment within 1e-11;
+    FP32 within 2e-5 of the FP64 increment (the single-RHS FP32 bar)."""
+    from cases import from_oracle, to_oracle
+    case = maxwell_case(n=n, N=N, m=3, bc=E.BC_PEC, flux=flux)
+    assert case.mesh.K % 16 == 2
+    x, y = case.omesh.x, case.omesh.y
+    # the cavity mode with 1% white noise: a smooth state alone has a tiny
+    # increment on the fine mesh, whose rounding floor would set the bar
+    q0 = E.tm_cavity_mode(x, y, 0.013, m=3)
+    q0 = np.ascontiguousarray(q0 + 1e-2 * np.random.default_rng(N).standard_normal(q0.shape))
+    dt = 0.2 / n / (2 * N + 1)
+    print("nodal LSERK4: N=%d K=%d" % (N, case.mesh.K), flush=True)
+    # against the oracle with the elements' exact affine geometric factors,
+    # as fp64_parity does (Apollo recomputes them at the cubature points, a
+    # rounding of its own of the same size as the bar at N = 5)
+    with O.exact_geometry():
+        ref = from_oracle(case.params.eqn, O.lserk4(case.omesh, case.ophys, 0.0, dt, steps,
+                                                    to_oracle(case.params.eqn, q0)))
+    out = {}
+    for prec in (E.PREC_FP64, E.PREC_FP32_MFMA):
+        p = E.Params(N, E.EQN_MAXWELL_TM3, flux=flux, precision=prec, bcs=[(E.BC_PEC, [])])
+        eng = E.Engine(case.mesh, p)
+        q = q0.copy()
+        eng.lserk4(0.0, dt, steps, q)
+        eng.close()
+        out[prec] = q
+    e64 = rel_l2(out[E.PREC_FP64] - q0, ref - q0)
+    e32 = rel_l2(out[E.PREC_FP32_MFMA] - q0, out[E.PREC_FP64] - q0)
+    print("  increment rel L2: FP64 vs oracle %.3e, FP32 vs FP64 %.3e" % (e64, e32), flush=True)
+    assert e64 <= 1e-11
+    assert e32 <= 2e-5
+
+
 def test_lserk4_device_pointer_and_time_dependent_bc():
     torch = pytest.importorskip("torch")
     case = euler_case(n=8, N=3, m=4, periodic=False)
