@@ -249,32 +249,55 @@ __device__ __forceinline__ unsigned phys_flux(const PhysParams &P, const S *q, S
 // Euler flux of one trace and its LF wave speed |u| + c
 // (wxeulereqn.cc:806-871, 1106-1110) from one reciprocal of the density and
 // one pressure; the density and pressure floors act as in phys_flux.
-template <int EQN>
-__device__ __forceinline__ unsigned euler_flux_speed(const PhysParams &P, const double *q,
-                                                     double *F, double *G, double *speed) {
+// Square roots of the LF wave speed: IEEE for FP64; the hardware's for FP32
+// (within an ulp), in the FP32 face flux of the FP32 and split-BF16x3 kernels.
+__device__ __forceinline__ double speed_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float speed_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+template <int EQN, typename S = double>
+__device__ __forceinline__ unsigned euler_flux_speed(const PhysParams &P, const S *q, S *F, S *G,
+                                                     S *speed) {
   constexpr bool W = (EQN == APL_EQN_EULER5);
   constexpr int IE = W ? 4 : 3;
   unsigned ef = 0;
-  double rho = q[0];
-  if (P.min_dens == 0.0 && !(rho > 0.0)) ef |= EF_DENSITY;
-  if (rho < P.min_dens) rho = P.min_dens;
-  const double ir = recip(rho);
-  const double u = q[1] * ir, v = q[2] * ir, w = W ? q[3] * ir : 0.0;
-  const double E = q[IE];
-  const double k2 = u * u + v * v + w * w;
-  double p = (P.gamma - 1.0) * (E - 0.5 * rho * k2);
-  if (!(p > 0.0)) {
+  S rho = q[0];
+  if (P.min_dens == 0.0 && !(rho > S(0))) ef |= EF_DENSITY;
+  if (rho < S(P.min_dens)) rho = S(P.min_dens);
+  const S ir = recip(rho);
+  const S u = q[1] * ir, v = q[2] * ir, w = W ? q[3] * ir : S(0);
+  const S E = q[IE];
+  const S k2 = u * u + v * v + w * w;
+  S p = S(P.gamma - 1.0) * (E - S(0.5) * rho * k2);
+  if (!(p > S(0))) {
     if (P.min_pres == 0.0) ef |= EF_PRESSURE;
-    else p = P.min_pres;
+    else p = S(P.min_pres);
   }
-  const double ru = rho * u, rv = rho * v;
+  const S ru = rho * u, rv = rho * v;
   F[0] = ru;         G[0] = rv;
   F[1] = ru * u + p; G[1] = rv * u;
   F[2] = ru * v;     G[2] = rv * v + p;
   if constexpr (W) { F[3] = ru * w; G[3] = rv * w; }
   F[IE] = (E + p) * u;
   G[IE] = (E + p) * v;
-  *speed = sqrt(k2) + sqrt(P.gamma * p * ir);
+  *speed = speed_sqrt(k2) + speed_sqrt(S(P.gamma) * p * ir);
+  return ef;
+}
+
+// The Euler LF flux in FP32 (wxeulereqn.cc:1088-1117): the face flux of the
+// FP32 and split-BF16x3 kernels, whose fluxes enter FP32 products anyway.
+template <int EQN>
+__device__ __forceinline__ unsigned euler_lf_f32(const PhysParams &P, float nx, float ny,
+                                                 const float *qM, const float *qP, float *fs,
+                                                 float *lam) {
+  constexpr int M = Vars<EQN>::M;
+  float FM[M], GM[M], FP[M], GP[M], cM, cP;
+  unsigned ef = euler_flux_speed<EQN, float>(P, qM, FM, GM, &cM);
+  ef |= euler_flux_speed<EQN, float>(P, qP, FP, GP, &cP);
+  const float l = cM > cP ? cM : cP;
+#pragma unroll
+  for (int c = 0; c < M; ++c)
+    fs[c] = 0.5f * (nx * (FM[c] + FP[c]) + ny * (GM[c] + GP[c]) + l * (qM[c] - qP[c]));
+  *lam = l;
   return ef;
 }
 
@@ -801,6 +824,69 @@ __device__ __forceinline__ void face_flux_cub(const RhsArgs &a, int ks, S *fsT, 
   constexpr int M = Vars<EQN>::M;
   constexpr int Ng = Order<N>::Ng, Nfp = Order<N>::Nfp;
   const int kf = 4 * ks + F.lg;
+  // FP32 face flux for the FP32 and split-BF16x3 Euler LF kernels up to
+  // N = 5 (C3: 444 -> 402 us per stage; at N = 8 it takes the single-RHS
+  // error past 2e-5, and there the volume dominates)
+  constexpr bool F32 = sizeof(S) == 4 && FLUX == APL_FLUX_LF && N <= 5 &&
+                       (EQN == APL_EQN_EULER5 || EQN == APL_EQN_EULER4);
+  if constexpr (F32) {
+    // FP32 traces and flux; the boundary state stays FP64
+    if (kf < 3 * Ng) {
+      const int f = kf / Ng, g = kf - f * Ng;
+      const int c0 = F.code[0], c1 = F.code[1], c2 = F.code[2];
+      const double x0 = F.nx[0], x1 = F.nx[1], x2 = F.nx[2];
+      const double y0 = F.ny[0], y1 = F.ny[1], y2 = F.ny[2];
+      const double s0 = F.sj[0], s1 = F.sj[1], s2 = F.sj[2];
+      const int code = f == 0 ? c0 : (f == 1 ? c1 : c2);
+      const double nx = f == 0 ? x0 : (f == 1 ? x1 : x2);
+      const double ny = f == 0 ? y0 : (f == 1 ? y1 : y2);
+      const double sj = f == 0 ? s0 : (f == 1 ? s1 : s2);
+      float qM[M], qP[M], fs[M];
+#pragma unroll
+      for (int v = 0; v < M; ++v) qM[v] = qP[v] = 0.0f;
+      const double *Io = F.I1 + (f * Ng + g) * Nfp;
+#pragma unroll
+      for (int i = 0; i < Nfp; ++i) {
+        const float w = float(Io[i]);
+        const XE *xo = F.XT + F.FM[f * Nfp + i] * M * XS + F.el;
+#pragma unroll
+        for (int v = 0; v < M; ++v) qM[v] += w * float(xo[v * XS]);
+      }
+      if (code >= 0) {
+        const int nf = code & 3;
+        const double *In = F.I1 + (nf * Ng + (Ng - 1 - g)) * Nfp;
+#pragma unroll
+        for (int i = 0; i < Nfp; ++i) {
+          const float w = float(In[Nfp - 1 - i]);
+#pragma unroll
+          for (int v = 0; v < M; ++v) qP[v] += w * float(F.PT[((f * Nfp + i) * M + v) * 16 + F.el]);
+        }
+      } else {
+        const double *vk = a.vert + F.ge * 6;
+        const int va = f, vb = (f + 1) % 3;
+        const double z = a.I1D[3 * Ng * Nfp + g];
+        const double sz = 0.5 * (1.0 + z);
+        const double xg = vk[2 * va] + sz * (vk[2 * vb] - vk[2 * va]);
+        const double yg = vk[2 * va + 1] + sz * (vk[2 * vb + 1] - vk[2 * va + 1]);
+        double dM[M], dP[M];
+#pragma unroll
+        for (int v = 0; v < M; ++v) dM[v] = qM[v];
+        boundary_state<EQN>(a.bc, -code - 1, F.tnow, xg, yg, nx, ny, dM, dP);
+#pragma unroll
+        for (int v = 0; v < M; ++v) qP[v] = float(dP[v]);
+      }
+      float lam;
+      F.ef |= euler_lf_f32<EQN>(a.ph, float(nx), float(ny), qM, qP, fs, &lam);
+      F.lmax = double(lam) > F.lmax ? double(lam) : F.lmax;
+      const float sjf = float(sj);
+#pragma unroll
+      for (int v = 0; v < M; ++v) fsT[v] = S(fs[v] * sjf);
+    } else {
+#pragma unroll
+      for (int v = 0; v < M; ++v) fsT[v] = S(0);
+    }
+    return;
+  }
   if (kf < 3 * Ng) {
     const int f = kf / Ng, g = kf - f * Ng;
     // selects of loaded values (a select of addresses would keep F in memory)
@@ -877,11 +963,15 @@ __device__ __forceinline__ void split_state(const XE *XT, int lg, int el, bf8 (&
 //  - FP64: two for Maxwell up to N = 6 (N = 6 303 -> 293), one for Euler
 //    above N = 4 and for N >= 7, where two waves spill (Euler N = 5
 //    346 -> 373, N = 7 637 -> 1380).
+//  - FP32 2D Euler (4 variables) at N = 4: four (127 VGPRs with the FP32
+//    face flux; C3 402 -> 380 us; at N = 3 four measured slower, C1 17.2 ->
+//    18.6 us; the 5-variable kernel would spill at four).
 template <int N, int EQN, typename T>
 struct MinWaves {
   static constexpr bool TM = EQN == APL_EQN_MAXWELL_TM3;
   static constexpr int value =
       EQN == APL_EQN_PHMAXWELL8 ? 1
+      : (sizeof(T) == 4 && N == 4 && EQN == APL_EQN_EULER4) ? 4
       : (N <= 4 || sizeof(T) == 4) ? 2
       : sizeof(T) == 1         ? (TM ? 2 : 1)
       : N <= 6                 ? (TM ? 2 : 1)
@@ -932,6 +1022,7 @@ __global__ void __launch_bounds__((64 * RhsGroup<N, EQN, T>::W), (MinWaves<N, EQ
   const long ge = e0 + (el < cnt ? el : 0);
   const double *gk = a.geom + ge * G_WORDS;
   const double jrx = gk[G_JRX], jry = gk[G_JRY], jsx = gk[G_JSX], jsy = gk[G_JSY];
+  const S jrx_s = S(jrx), jry_s = S(jry), jsx_s = S(jsx), jsy_s = S(jsy);
   const double Jel = gk[G_J];
   const double nx0 = gk[G_NX], nx1 = gk[G_NX + 1], nx2 = gk[G_NX + 2];
   const double ny0 = gk[G_NY], ny1 = gk[G_NY + 1], ny2 = gk[G_NY + 2];
@@ -1066,10 +1157,20 @@ __global__ void __launch_bounds__((64 * RhsGroup<N, EQN, T>::W), (MinWaves<N, EQ
 #pragma unroll
         for (int v = 0; v < M; ++v) qv[v] = qc[v][r];
         ef |= phys_flux<EQN, S>(a.ph, qv, F, G);
+        if constexpr (sizeof(S) == 4) {
+          // FP32 kernels: the scaled fluxes enter FP32 products, so they are
+          // formed in FP32
 #pragma unroll
-        for (int v = 0; v < M; ++v) {
-          phr[v] = S(jrx * double(F[v]) + jry * double(G[v]));
-          phs[v] = S(jsx * double(F[v]) + jsy * double(G[v]));
+          for (int v = 0; v < M; ++v) {
+            phr[v] = jrx_s * F[v] + jry_s * G[v];
+            phs[v] = jsx_s * F[v] + jsy_s * G[v];
+          }
+        } else {
+#pragma unroll
+          for (int v = 0; v < M; ++v) {
+            phr[v] = S(jrx * double(F[v]) + jry * double(G[v]));
+            phs[v] = S(jsx * double(F[v]) + jsy * double(G[v]));
+          }
         }
       } else {
 #pragma unroll

@@ -14,7 +14,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libapollo_dg.so")
+# the engine library: the in-tree build, or another build of the same
+# sources named by APOLLO_DG_LIB (for side-by-side timing of two builds)
+LIB_PATH = os.environ.get("APOLLO_DG_LIB") or os.path.join(_HERE, "libapollo_dg.so")
 
 # enums (include/apollo_dg.h)
 APL_OK = 0
