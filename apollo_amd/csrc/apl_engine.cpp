@@ -37,6 +37,7 @@ struct apl_handle_s {
   apl_params params;
   int K = 0, M = 0, Nv = 0;
   bool linear = false;   // nodal linear-flux kernel (Maxwell, APL_ALGO_AUTO)
+  bool valu = false;     // cubature form on the VALU kernel (N <= 3)
   Operators ops;
   Geometry geo;
   std::vector<int> etetf, fcode, vmapP;
@@ -149,6 +150,13 @@ static int dev_alloc(apl_handle *h, void **dst, size_t bytes) {
   return APL_OK;
 }
 
+// The cubature form at N <= 3 (FP64 and FP32) runs on the VALU kernel
+// unless APL_ALGO_CUBATURE_MFMA asks for the MFMA kernel: measured faster
+// on the 2D Euler vortex at N = 3 (us per fused stage, VALU vs MFMA):
+// K = 8192 FP64 18.4 vs 20.4, FP32 15.4 vs 16.8; K = 131072 FP64 96.2 vs
+// 106.4, FP32 67.7 vs 75.2.
+constexpr bool VALU_BY_DEFAULT = true;
+
 static int nvars(int eqn) {
   switch (eqn) {
     case APL_EQN_EULER5: return 5;
@@ -162,7 +170,8 @@ static int nvars(int eqn) {
 static int dispatch_rhs(apl_handle *h, const RhsArgs &a) {
   const int nb = (h->K + 15) / 16;
   const apl_params &p = h->params;
-  const int lin = h->linear ? 1 : 0;
+  // kernel: 0 cubature form on MFMA, 1 nodal form, 2 cubature form on VALU
+  const int lin = h->linear ? 1 : (h->valu ? 2 : 0);
   int rc;
   switch (p.order) {
     case 1: rc = launch_rhs<1>(p.eqn, p.flux, p.precision, lin, h->stream, a, nb); break;
@@ -224,15 +233,14 @@ static int check_flags(apl_handle *h) {
   return flags_status(h, *pf);
 }
 
-// Fold and permute the operators into MFMA fragments (see apl_device.h).
-// LT is the operand type whose accumulator layout the fragments follow.
-template <typename T, typename LT = T>
-static void make_fragments(const Operators &o, std::vector<T> &A1, std::vector<T> &A2,
-                           std::vector<T> &A3) {
+// The folded operators of the cubature form: A_r = V V^T Dr_c^T diag(w)
+// and A_s likewise [Np x Nc], L = V V^T I_f^T diag(gw) [Np x 3Ng].
+static void fold_operators(const Operators &o, std::vector<double> &Ar, std::vector<double> &As,
+                           std::vector<double> &L) {
   const int Np = o.Np, Nc = o.Nc, Ng = o.Ng;
-  const int NKS1 = (Np + 3) / 4, NCH = (Nc + 15) / 16, NRT = (Np + 15) / 16, NKF = (3 * Ng + 3) / 4;
-  // A_r = V V^T Dr_c^T diag(w), A_s likewise, L = V V^T I_f^T diag(gw)
-  std::vector<double> Ar((size_t)Np * Nc, 0.0), As((size_t)Np * Nc, 0.0), L((size_t)Np * 3 * Ng, 0.0);
+  Ar.assign((size_t)Np * Nc, 0.0);
+  As.assign((size_t)Np * Nc, 0.0);
+  L.assign((size_t)Np * 3 * Ng, 0.0);
   for (int n = 0; n < Np; ++n) {
     for (int c = 0; c < Nc; ++c) {
       double sr = 0.0, ss = 0.0;
@@ -249,6 +257,17 @@ static void make_fragments(const Operators &o, std::vector<T> &A1, std::vector<T
       L[(size_t)n * 3 * Ng + k] = s * o.gw[k % Ng];
     }
   }
+}
+
+// Fold and permute the operators into MFMA fragments (see apl_device.h).
+// LT is the operand type whose accumulator layout the fragments follow.
+template <typename T, typename LT = T>
+static void make_fragments(const Operators &o, std::vector<T> &A1, std::vector<T> &A2,
+                           std::vector<T> &A3) {
+  const int Np = o.Np, Nc = o.Nc, Ng = o.Ng;
+  const int NKS1 = (Np + 3) / 4, NCH = (Nc + 15) / 16, NRT = (Np + 15) / 16, NKF = (3 * Ng + 3) / 4;
+  std::vector<double> Ar, As, L;
+  fold_operators(o, Ar, As, L);
   A1.assign((size_t)NCH * NKS1 * 64, T(0));
   for (int cc = 0; cc < NCH; ++cc)
     for (int ks = 0; ks < NKS1; ++ks)
@@ -275,6 +294,25 @@ static void make_fragments(const Operators &o, std::vector<T> &A1, std::vector<T
         const int n = 16 * rt + (l & 15), k = 4 * ks + (l >> 4);
         if (n < Np && k < 3 * Ng) A3[((size_t)rt * NKF + ks) * 64 + l] = T(-L[(size_t)n * 3 * Ng + k]);
       }
+}
+
+// Point tables of the VALU cubature kernel (apl_valu.h): row c of A1 holds
+// V_c[c, 0..Np), A_r[0..Np, c), A_s[0..Np, c); row k of A3 holds -L[0..Np, k).
+template <typename T>
+static void make_valu_tables(const Operators &o, std::vector<T> &A1, std::vector<T> &A3) {
+  const int Np = o.Np, Nc = o.Nc, Ng = o.Ng;
+  std::vector<double> Ar, As, L;
+  fold_operators(o, Ar, As, L);
+  A1.assign((size_t)Nc * 3 * Np, T(0));
+  for (int c = 0; c < Nc; ++c)
+    for (int n = 0; n < Np; ++n) {
+      A1[(size_t)c * 3 * Np + n] = T(o.Vc[(size_t)c * Np + n]);
+      A1[(size_t)c * 3 * Np + Np + n] = T(Ar[(size_t)n * Nc + c]);
+      A1[(size_t)c * 3 * Np + 2 * Np + n] = T(As[(size_t)n * Nc + c]);
+    }
+  A3.assign((size_t)3 * Ng * Np, T(0));
+  for (int k = 0; k < 3 * Ng; ++k)
+    for (int n = 0; n < Np; ++n) A3[(size_t)k * Np + n] = T(-L[(size_t)n * 3 * Ng + k]);
 }
 
 // Fragments of the nodal linear-flux kernel: A1 = [Dt_r; Dt_s] with
@@ -343,6 +381,9 @@ static int upload_fragments(apl_handle *h) {
   std::vector<T> A1, A2, A3;
   if (h->linear) {
     make_linear_fragments<T>(h->ops, A1, A3);
+    A2.assign(1, T(0));
+  } else if (h->valu) {
+    make_valu_tables<T>(h->ops, A1, A3);
     A2.assign(1, T(0));
   } else {
     make_fragments<T>(h->ops, A1, A2, A3);
@@ -544,8 +585,12 @@ int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out)
   if (params->precision != APL_PREC_FP64 && params->precision != APL_PREC_FP32_MFMA &&
       params->precision != APL_PREC_BF16X3)
     return fail(nullptr, APL_ERR_ARG, "unknown precision");
-  if (params->algorithm != APL_ALGO_AUTO && params->algorithm != APL_ALGO_CUBATURE)
+  if (params->algorithm < APL_ALGO_AUTO || params->algorithm > APL_ALGO_CUBATURE_VALU)
     return fail(nullptr, APL_ERR_ARG, "unknown algorithm");
+  if (params->algorithm == APL_ALGO_CUBATURE_VALU &&
+      (params->order > 3 || params->precision == APL_PREC_BF16X3))
+    return fail(nullptr, APL_ERR_UNSUPPORTED,
+                "the VALU cubature kernel covers N <= 3 in FP64 and FP32");
   if (params->eqn == APL_EQN_MAXWELL_TM3 && (params->chi != 0.0 || params->gamma_ph != 0.0))
     return fail(nullptr, APL_ERR_ARG, "MAXWELL_TM3 requires chi == gamma == 0");
   if (params->n_bc < 0 || params->n_bc > APL_MAX_BC) return fail(nullptr, APL_ERR_ARG, "n_bc");
@@ -573,6 +618,9 @@ int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out)
   h->params = *params;
   h->linear = params->algorithm == APL_ALGO_AUTO &&
               (params->eqn == APL_EQN_MAXWELL_TM3 || params->eqn == APL_EQN_PHMAXWELL8);
+  h->valu = !h->linear && params->order <= 3 && params->precision != APL_PREC_BF16X3 &&
+            (params->algorithm == APL_ALGO_CUBATURE_VALU ||
+             (params->algorithm != APL_ALGO_CUBATURE_MFMA && VALU_BY_DEFAULT));
   h->K = mesh->K;
   h->M = M;
   h->Nv = mesh->Nv;

@@ -4,6 +4,7 @@
 #include "apl_adjoint.h"
 #include "apl_device.h"
 #include "apl_limiter.h"
+#include "apl_valu.h"
 
 #ifndef APL_ORDER
 #error "compile with -DAPL_ORDER=<N>"
@@ -28,25 +29,46 @@ static int launch_batches(K kern, hipStream_t s, const A &a, int nbatches) {
   return hipGetLastError() == hipSuccess ? APL_OK : APL_ERR_HIP;
 }
 
+// kind: 0 the cubature form on MFMA, 1 the nodal form (linear fluxes),
+// 2 the cubature form on the VALU kernel (N <= 3, FP64 and FP32)
 template <int N, int EQN, int FLUX, typename T>
-static int launch_typed(int linear, hipStream_t s, const RhsArgs &a, int nblocks) {
+static int launch_typed(int kind, hipStream_t s, const RhsArgs &a, int nblocks) {
   if constexpr (EQN == APL_EQN_PHMAXWELL8 || EQN == APL_EQN_MAXWELL_TM3) {
-    if (linear)
+    if (kind == 1)
       return launch_batches<LinLds<N, EQN, T>::doubles, LinGroup<N, EQN, T>::W>(
-          rhs_linear_kernel<N, EQN, FLUX, T>, s, a,
-                                                         nblocks);
+          rhs_linear_kernel<N, EQN, FLUX, T>, s, a, nblocks);
   }
-  if (linear) return APL_ERR_ARG;
+  if (kind == 1) return APL_ERR_ARG;
+  if (kind == 2) {
+    if constexpr (N <= 3 && sizeof(T) != 1) {
+      // 64 elements per workgroup of four waves, or of eight on small meshes
+      const int groups = (a.K + 63) / 64;
+      if (groups < VALU_SMALL_GROUPS && ValuLds<N, EQN, T, 8>::doubles * 8 <= 163840) {
+        auto kern = rhs_valu_kernel<N, EQN, FLUX, T, 8>;
+        const size_t lds = sizeof(double) * (size_t)ValuLds<N, EQN, T, 8>::doubles;
+        if (!allow_lds(kern, lds)) return APL_ERR_UNSUPPORTED;
+        hipLaunchKernelGGL(kern, dim3(groups), dim3(512), lds, s, a);
+      } else {
+        auto kern = rhs_valu_kernel<N, EQN, FLUX, T, 4>;
+        const size_t lds = sizeof(double) * (size_t)ValuLds<N, EQN, T, 4>::doubles;
+        if (!allow_lds(kern, lds)) return APL_ERR_UNSUPPORTED;
+        hipLaunchKernelGGL(kern, dim3(groups), dim3(256), lds, s, a);
+      }
+      return hipGetLastError() == hipSuccess ? APL_OK : APL_ERR_HIP;
+    } else {
+      return APL_ERR_UNSUPPORTED;
+    }
+  }
   return launch_batches<Lds<N, EQN, T>::doubles, RhsGroup<N, EQN, T>::W>(rhs_kernel<N, EQN, FLUX, T>, s,
                                                                           a, nblocks);
 }
 
 template <int N, int EQN, int FLUX>
-static int launch_prec(int prec, int linear, hipStream_t s, const RhsArgs &a, int nblocks) {
+static int launch_prec(int prec, int kind, hipStream_t s, const RhsArgs &a, int nblocks) {
   switch (prec) {
-    case APL_PREC_FP64: return launch_typed<N, EQN, FLUX, double>(linear, s, a, nblocks);
-    case APL_PREC_FP32_MFMA: return launch_typed<N, EQN, FLUX, float>(linear, s, a, nblocks);
-    case APL_PREC_BF16X3: return launch_typed<N, EQN, FLUX, Bf16x3>(linear, s, a, nblocks);
+    case APL_PREC_FP64: return launch_typed<N, EQN, FLUX, double>(kind, s, a, nblocks);
+    case APL_PREC_FP32_MFMA: return launch_typed<N, EQN, FLUX, float>(kind, s, a, nblocks);
+    case APL_PREC_BF16X3: return launch_typed<N, EQN, FLUX, Bf16x3>(kind, s, a, nblocks);
     default: return APL_ERR_UNSUPPORTED;
   }
 }

@@ -77,6 +77,9 @@ VARIANTS = {
     # name: (precision, algorithm)
     "fp64": (0, 0), "fp32": (1, 0), "bf16x3": (2, 0),
     "fp64_cub": (0, 1), "fp32_cub": (1, 1), "bf16x3_cub": (2, 1),
+    # the cubature form on a named kernel: MFMA at every order, or the
+    # vector-ALU kernel (N <= 3)
+    "fp64_mfma": (0, 2), "fp32_mfma": (1, 2), "fp64_valu": (0, 3), "fp32_valu": (1, 3),
 }
 
 
@@ -427,11 +430,19 @@ def calibrate_copy(mib, reps=10):
 
 
 def kernel_name(w, variant):
-    from apollo_amd import engine as E
+    """The engine's kernel for a variant: the nodal kernel for linear fluxes
+    (algorithm 0), the vector-ALU cubature kernel at N <= 3 in FP64/FP32
+    (algorithms 0, 1, 3; eight waves per group below 512 groups), else the
+    MFMA cubature kernel."""
     N = w.order
-    lin = w.linear and VARIANTS[variant][1] == 0
+    prec, algo = VARIANTS[variant]
     ty = {"fp64": "double", "fp32": "float", "bf16x3": "apl::Bf16x3"}[variant.split("_")[0]]
-    return "%s<%d,%d,%d,%s>" % ("rhs_linear_kernel" if lin else "rhs_kernel", N, w.eqn, w.flux, ty)
+    if w.linear and algo == 0:
+        return "rhs_linear_kernel<%d,%d,%d,%s>" % (N, w.eqn, w.flux, ty)
+    if N <= 3 and prec != 2 and algo != 2:
+        nw = 8 if (w.K + 63) // 64 < 512 else 4
+        return "rhs_valu_kernel<%d,%d,%d,%s,%d>" % (N, w.eqn, w.flux, ty, nw)
+    return "rhs_kernel<%d,%d,%d,%s>" % (N, w.eqn, w.flux, ty)
 
 
 def main():
@@ -447,7 +458,9 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the fixed 500-step accuracy run of the headline workload")
     ap.add_argument("--only", default="",
-                    help="time one variant only (fp64, fp32, bf16x3, fp64_cub, fp32_cub, bf16x3_cub)")
+                    help="time these variants only, comma-separated (fp64, fp32, bf16x3, "
+                         "fp64_cub, fp32_cub, bf16x3_cub, fp64_mfma, fp32_mfma, fp64_valu, "
+                         "fp32_valu)")
     ap.add_argument("--limiter-only", action="store_true", help="time the limiter legs only")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run a plain MIB-MiB device copy instead (counter calibration)")
@@ -469,7 +482,7 @@ def main():
     w = W.get(args.workload)
     names = ["fp64", "fp32", "bf16x3"] + (["fp64_cub", "fp32_cub", "bf16x3_cub"] if w.linear else [])
     if args.only:
-        names = [args.only]
+        names = args.only.split(",")
     res = {}
     for v in names:
         print("bench: %s %s steps=%d" % (args.workload, v, args.steps), file=sys.stderr, flush=True)

@@ -90,9 +90,18 @@ enum apl_algorithm {
                               linear fluxes (Maxwell) the algebraically
                               identical nodal form V V^T Dr_c^T W V_c Q ... on
                               straight-sided elements (no reference
-                              counterpart; same result to rounding)          */
-  APL_ALGO_CUBATURE = 1    /* always Apollo's over-integrated cubature form
-                              (wxcubature2d.cc:288-382)                      */
+                              counterpart; same result to rounding).  The
+                              cubature form runs on the vector-ALU kernel at
+                              N <= 3 in FP64 and FP32, on MFMA otherwise     */
+  APL_ALGO_CUBATURE = 1,   /* always Apollo's over-integrated cubature form
+                              (wxcubature2d.cc:288-382), on the kernels
+                              APL_ALGO_AUTO picks for it                     */
+  APL_ALGO_CUBATURE_MFMA = 2, /* the cubature form on the MFMA kernel at every
+                              order (padded 16x16x4 tiles at N <= 3)         */
+  APL_ALGO_CUBATURE_VALU = 3  /* the cubature form on the vector-ALU kernel
+                              (one lane per element, operators as scalar
+                              operands); N <= 3, FP64 or FP32 only, else
+                              APL_ERR_UNSUPPORTED                            */
 };
 
 enum apl_bc_kind {
