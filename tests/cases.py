@@ -145,6 +145,49 @@ def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
 
 
+def frame_sensitivity(case, q0, t1, dt0, cfl, qo, dto, seeds=(1, 2, 3), centred=True):
+    """How far the oracle's own FP64 roundings move one output frame of
+    Apollo's stepping (O.ssp_advance over [0, t1] from q0), as the largest
+    relative L2 change of the increment qo - q0 and the largest relative
+    change of a dt, over
+      - a random one-ulp relative perturbation of q0 (each seed);
+      - the oracle with the elements' exact affine geometric factors
+        (Apollo recomputes them at the cubature points, quirk q8; the
+        engine uses the affine constants);
+      - with `centred`, the oracle on the mesh translated to be centred on
+        the origin (Apollo's limiter forms absolute node coordinates and
+        subtracts their average, a cancellation of size eps |x| / h; the
+        engine's reconstruction works with coordinate offsets and does not
+        see it).  Only for problems whose boundary states do not depend on
+        position.
+    Returns (sensitivity, dt sensitivity, {kind: (increment, dt)})."""
+    eqn = case.params.eqn
+    qo0 = to_oracle(eqn, q0)
+    inc = qo - q0
+    parts = {}
+
+    def record(kind, q, dts):
+        n = min(len(dts), len(dto))
+        parts[kind] = (rel_l2(from_oracle(eqn, q) - q0, inc),
+                       float(np.abs(dts[:n] - dto[:n]).max() / dto.max()) if n else 0.0)
+
+    for seed in seeds:
+        rng = np.random.default_rng(seed)
+        record("seed %d" % seed, *O.ssp_advance(case.omesh, case.ophys, 0.0, t1, dt0, cfl,
+                                                qo0 * (1.0 + 2.2e-16 * rng.standard_normal(qo0.shape)),
+                                                max_steps=20000))
+    with O.exact_geometry():
+        record("exact geometry", *O.ssp_advance(case.omesh, case.ophys, 0.0, t1, dt0, cfl, qo0,
+                                                max_steps=20000))
+    if centred:
+        m = case.mesh
+        cx, cy = 0.5 * (m.VX.min() + m.VX.max()), 0.5 * (m.VY.min() + m.VY.max())
+        om = O.Mesh(case.ops, m.VX - cx, m.VY - cy, m.EToV, m.vert_id, m.face_tag)
+        record("centred mesh", *O.ssp_advance(om, case.ophys, 0.0, t1, dt0, cfl, qo0,
+                                              max_steps=20000))
+    return max(v[0] for v in parts.values()), max(v[1] for v in parts.values()), parts
+
+
 # --- standard meshes ------------------------------------------------------
 
 def periodic_square(n, L=10.0, jitter=0.1, seed=0):

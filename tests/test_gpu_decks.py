@@ -8,19 +8,24 @@ enables it) over one output frame of a few steps.  The decks' own meshes,
 except the backward-facing step's grid2.msh, for which its grid.msh stands
 in (same geometry and tags, a quarter of the triangles).
 
-Bar: the engine's increment q(t) - q(0) within max(1e-11, 10 x the
-oracle's own sensitivity) in relative L2, where the sensitivity is how far a
-one-ulp relative perturbation of the initial state moves the oracle's
-increment; the same number of steps, and each dt within the same bar
-(relative to the largest dt), its sensitivity measured the same way.  The discontinuous starts (backward step, explosion)
-and the limiter's gradient weights make that sensitivity the honest scale."""
+Bar: the engine's increment q(t) - q(0) within max(1e-12, 10 x the
+oracle's own sensitivity) in relative L2, and each dt within max(1e-12,
+10 x the oracle's dt sensitivity) of the largest dt, with the same number
+of steps.  The sensitivity is the largest change of the oracle's frame
+under its own FP64 roundings (cases.frame_sensitivity): one-ulp input
+perturbations (three seeds), Apollo's recomputed geometric factors against
+the affine ones, and the mesh translated to the origin, which moves the
+output of Apollo's limiter (it subtracts averages of absolute node
+coordinates) by as much as the engine differs from it: on the explosion at
+N = 2, the limiter's change to the initial state moves by 1.2e-14 (relative
+L2) under the translation, and the engine's differs by 1.2e-14."""
 import numpy as np
 import pytest
 
 from apollo_amd import engine as E
 from oracle import oracle as O
 
-from cases import DECKS, deck_case, deck_state, from_oracle, rel_l2, to_oracle
+from cases import DECKS, deck_case, deck_state, frame_sensitivity, from_oracle, rel_l2, to_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -45,22 +50,58 @@ def test_shock_deck_matches_oracle(name, nsteps, N):
     eng.close()
     qo, dto = O.ssp_advance(case.omesh, case.ophys, 0.0, t1, dt0, cfl, qo0)
     qo = from_oracle(case.params.eqn, qo)
-    rng = np.random.default_rng(1)
-    qp, dtp = O.ssp_advance(case.omesh, case.ophys, 0.0, t1, dt0, cfl,
-                            qo0 * (1.0 + 2.2e-16 * rng.standard_normal(qo0.shape)))
-    sens = rel_l2(from_oracle(case.params.eqn, qp) - q0, qo - q0)
-    n = min(len(dtp), len(dto))
-    sens_dt = np.abs(dtp[:n] - dto[:n]).max() / dto.max()
+    sens, sens_dt, parts = frame_sensitivity(case, q0, t1, dt0, cfl, qo, dto)
     err = rel_l2(q - q0, qo - q0)
-    print("%s N=%d (K=%d, limiter %s): %d steps, increment rel L2 vs oracle %.3e, "
-          "oracle's one-ulp sensitivity %.3e"
-          % (name, N, case.mesh.K, "on" if DECKS[name][4] else "off", len(dts), err, sens))
+    ddt = np.abs(dts - dto).max() / dto.max() if len(dts) == len(dto) else np.nan
+    print("%s N=%d (K=%d, limiter %s): %d steps, increment rel L2 vs oracle %.3e, dt max diff "
+          "%.3e; oracle sensitivities (increment, dt): %s"
+          % (name, N, case.mesh.K, "on" if DECKS[name][4] else "off", len(dts), err, ddt,
+             ", ".join("%s %.2e %.2e" % (k, *v) for k, v in parts.items())), flush=True)
     # the frame ends at (nsteps - 1/2) dt0; dt shrinks under the dt control
     # when the wave speeds grow (the explosion), so there may be more steps
     assert len(dts) == len(dto) >= nsteps
-    # dt follows from the state's largest wave speed, so it is held to the
-    # state's bar (measured: the explosion at N = 2 differs by 1.4e-12)
-    assert np.abs(dts - dto).max() <= max(1e-11, 10.0 * sens_dt) * dto.max()
-    assert err <= max(1e-11, 10.0 * sens)
+    # dt follows from the state's largest wave speed
+    assert ddt <= max(1e-12, 10.0 * sens_dt)
+    assert err <= max(1e-12, 10.0 * sens)
     assert rel_l2(q, q0) > 1e-4                              # the steps did act
     assert np.isfinite(q).all() and (q[..., 0] > 0).all()
+
+
+@pytest.mark.parametrize("N", [1, 2])
+def test_explosion_first_stage_components(N):
+    """The explosion deck's first stage piece by piece against the oracle,
+    to place where its increment departs from the oracle by more than the
+    oracle's own sensitivities: the limiter's output on the initial state,
+    the RHS of the limited state, the suggested dt (the LF speed maximum),
+    then the increments after 1, 2 and 4 steps."""
+    case = deck_case("explosion", N)
+    q0 = deck_state("explosion", case)
+    eqn = case.params.eqn
+    qo0 = to_oracle(eqn, q0)
+    cfl = 1.0 / (2 * (N + 1) - 1)
+    eng = E.Engine(case.mesh, case.params)
+    ql = np.zeros_like(q0)
+    eng.limit(q0, ql)
+    qlo = from_oracle(eqn, O.limit(case.omesh, case.ophys, qo0))
+    e_lim = rel_l2(ql - q0, qlo - q0)
+    m_lim = np.abs(ql - qlo).max() / np.abs(qlo).max()
+    out = np.zeros_like(q0)
+    dt_e = eng.rhs(0.0, qlo, out, cfl)
+    ref, ms = case.oracle_rhs(qlo)
+    dt_o = O.suggested_dt(case.omesh, cfl, ms)
+    with O.exact_geometry():
+        ref_x, _ = case.oracle_rhs(qlo)
+    e_rhs, e_rhs_x = rel_l2(out, ref), rel_l2(out, ref_x)
+    print("explosion N=%d: limiter change rel L2 %.3e (max %.3e); RHS of the limited state %.3e "
+          "(exact-geometry oracle %.3e); suggested dt rel diff %.3e"
+          % (N, e_lim, m_lim, e_rhs, e_rhs_x, abs(dt_e - dt_o) / dt_o), flush=True)
+    dt0 = O.initial_dt(case.omesh, case.ophys, qo0, cfl)
+    for nsteps in (1, 2, 4):
+        t1 = (nsteps - 0.5) * dt0
+        q = q0.copy()
+        eng.ssp_advance(0.0, t1, dt0, cfl, q)
+        qo, _ = O.ssp_advance(case.omesh, case.ophys, 0.0, t1, dt0, cfl, qo0)
+        print("  %d step(s): increment rel L2 vs oracle %.3e"
+              % (nsteps, rel_l2(q - q0, from_oracle(eqn, qo) - q0)), flush=True)
+    eng.close()
+    assert m_lim <= 1e-13 and e_rhs_x <= 1e-12 and abs(dt_e - dt_o) <= 1e-13 * dt_o
