@@ -12,9 +12,13 @@
 //                              whose exterior state is this element's trace)
 //                            [+ (dqBC/dq-)^T (dF*/dq+)^T Sb sJ on a boundary]
 // Kernel A writes Sb for every element; kernel B gathers the neighbours' Sb,
-// so every element writes only its own output (no atomics).  Kernel A also
-// folds the reverse-LSERK4 update rbar += b qbar, kernel B the update
-// qbar += dt out, rbar *= a.
+// so every element writes only its own output (no atomics).  In a reverse
+// LSERK4 stage lam = rbar + b qbar, and kernel B applies the updates
+// qbar += dt out, rbar = a lam; it then forms the next reverse stage's
+// lam' = rbar + b' qbar and writes that stage's Sb into a second buffer
+// (its own element only), so that each reverse stage after the first is one
+// launch of kernel B: the neighbours' Sb it reads were written by the
+// previous launch.
 #pragma once
 
 #include "apl_device.h"
@@ -40,8 +44,11 @@ struct AdjArgs {
   const double *Avct;       // V_c^T fragments      [NRT][NCH][64][4] (acc_row permuted)
   const double *Alt;        // -L^T fragments       [NFT][NKS1][64]
   const double *Aift;       // I_f^T fragments      [NRT][NKF][64]
+  double *sbar_out;         // mode 1: the next stage's Sb (kernel B), or null
+  int face_pass;            // 1: kernel A computes Sb before kernel B runs
   double t;                 // time of the stage (boundary states)
   double rk_a, rk_b, dt;    // reverse stage coefficients (mode 1)
+  double rk_b_next;         // mode 1: b of the next reverse stage
   unsigned *err;
   PhysParams ph;
   BCTable bc;
@@ -207,11 +214,10 @@ __device__ __forceinline__ void boundary_vjp(const BCTable &B, int idx, double n
   for (int c = 0; c < M; ++c) qMb[c] += t[c];
 }
 
-// y[i] += b x[i] over one batch of 16 elements of ES doubles, written back,
-// and the updated values staged transposed as stage_transposed does
-// (XT[w*16 + e]; missing elements repeat element 0).
+// y[i] + b x[i] over one batch of 16 elements of ES doubles, staged
+// transposed as stage_transposed does (XT[w*16 + e]; zero past the batch).
 template <int ES>
-__device__ __forceinline__ void axpy_stage_transposed(double *XT, double *y, const double *x,
+__device__ __forceinline__ void axpy_stage_transposed(double *XT, const double *y, const double *x,
                                                       double b, int cnt, int lane) {
   constexpr int NV2 = 16 * ES / 2, NIT = (NV2 + 63) / 64;
   if (cnt == 16) {
@@ -229,7 +235,6 @@ __device__ __forceinline__ void axpy_stage_transposed(double *XT, double *y, con
       const int idx = lane + 64 * j;
       if (idx < NV2) {
         const dbl2 v = yv[j] + b * xv[j];
-        ((dbl2 *)y)[idx] = v;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const int i = 2 * idx + h, e = i / ES, w = i - e * ES;
@@ -241,9 +246,7 @@ __device__ __forceinline__ void axpy_stage_transposed(double *XT, double *y, con
     for (int i = lane; i < 16 * ES; i += 64) {
       const int e = i / ES, w = i - e * ES;
       if (e < cnt) {
-        const double v = y[i] + b * x[i];
-        y[i] = v;
-        XT[w * 16 + e] = v;
+        XT[w * 16 + e] = y[i] + b * x[i];
       } else {
         XT[w * 16 + e] = 0.0;
       }
@@ -267,8 +270,52 @@ __device__ __forceinline__ void copy_batch(double *dst, const double *src, int c
 }
 
 // ---------------------------------------------------------------------------
-// kernel A: lh = lam / J (mode 1: rbar += b qbar first), Sb = -L^T lh
+// kernel A: lh = lam / J (mode 1: lam = rbar + b qbar), Sb = -L^T lh
 // ---------------------------------------------------------------------------
+
+// Sb = -L^T (lam / J) for one batch: LT holds lam transposed (rows >= Np
+// are zeroed here), sc is 1/J of lane el's element (0 past the batch end);
+// SO (LDS, 16 FS doubles) stages the result, which is written to dst.
+template <int N, int EQN>
+__device__ __forceinline__ void face_weights(const AdjArgs &a, double *LT, double *SO, double sc,
+                                             double *dst, int cnt, int lane) {
+  using O = Order<N>;
+  constexpr int M = Vars<EQN>::M;
+  constexpr int Np = O::Np, Ng = O::Ng, NKS1 = O::NKS1, NFT = AdjOrder<N>::NFT;
+  constexpr int ES = Np * M, FS = 3 * Ng * M;
+  typedef typename Acc<double>::type V4;
+  const int el = lane & 15, lg = lane >> 4;
+  for (int i = 16 * ES + lane; i < NKS1 * 4 * M * 16; i += 64) LT[i] = 0.0;
+  wave_sync();
+  double b[NKS1][M];
+#pragma unroll
+  for (int ks = 0; ks < NKS1; ++ks)
+#pragma unroll
+    for (int v = 0; v < M; ++v) b[ks][v] = LT[((4 * ks + lg) * M + v) * 16 + el] * sc;
+  wave_sync();   // SO may overlay LT's space in the caller
+#pragma unroll
+  for (int ft = 0; ft < NFT; ++ft) {
+    V4 acc[M];
+#pragma unroll
+    for (int v = 0; v < M; ++v) acc[v] = V4{0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < NKS1; ++ks) {
+      const double av = a.Alt[(ft * NKS1 + ks) * 64 + lane];
+#pragma unroll
+      for (int v = 0; v < M; ++v) acc[v] = mma(av, b[ks][v], acc[v]);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int kf = 16 * ft + acc_row<double>(lg, r);
+      if (kf < 3 * Ng) {
+#pragma unroll
+        for (int v = 0; v < M; ++v) SO[el * FS + kf * M + v] = acc[v][r];
+      }
+    }
+  }
+  wave_sync();
+  copy_batch<FS>(dst, SO, cnt, lane);
+}
 
 template <int N, int EQN>
 struct AdjFaceLds {
@@ -301,44 +348,16 @@ __global__ void __launch_bounds__((64 * AdjFaceGroup<N, EQN>::W)) adj_face_kerne
   const int lane = wb.lane, el = lane & 15, lg = lane >> 4;
   const long e0 = wb.e0;
   const int cnt = wb.cnt;
-  double *rb = (double *)a.lam;
-  // mode 1: rbar += b qbar (written back); LT = rbar transposed.  1/J is
-  // applied to the B operands (lane el's own element; 0 past the batch end).
+  // mode 1: LT = (rbar + b qbar) transposed; mode 0: lam transposed.  1/J
+  // is applied to the B operands (lane el's own element; 0 past the batch
+  // end).
   const long ge = e0 + (el < cnt ? el : 0);
   const double sc = el < cnt ? a.geom[ge * G_WORDS + G_INVJ] : 0.0;
-  if (a.mode == 1 && wb.active)
-    axpy_stage_transposed<ES>(LT, rb + e0 * ES, a.qbar + e0 * ES, a.rk_b, cnt, lane);
+  if (a.mode == 1)
+    axpy_stage_transposed<ES>(LT, a.lam + e0 * ES, a.qbar + e0 * ES, a.rk_b, cnt, lane);
   else
-    stage_transposed<ES, 16>(LT, rb + e0 * ES, cnt, lane);
-  for (int i = 16 * ES + lane; i < NKS1 * 4 * M * 16; i += 64) LT[i] = 0.0;
-  wave_sync();
-  double b[NKS1][M];
-#pragma unroll
-  for (int ks = 0; ks < NKS1; ++ks)
-#pragma unroll
-    for (int v = 0; v < M; ++v) b[ks][v] = LT[((4 * ks + lg) * M + v) * 16 + el] * sc;
-#pragma unroll
-  for (int ft = 0; ft < NFT; ++ft) {
-    V4 acc[M];
-#pragma unroll
-    for (int v = 0; v < M; ++v) acc[v] = V4{0, 0, 0, 0};
-#pragma unroll
-    for (int ks = 0; ks < NKS1; ++ks) {
-      const double av = a.Alt[(ft * NKS1 + ks) * 64 + lane];
-#pragma unroll
-      for (int v = 0; v < M; ++v) acc[v] = mma(av, b[ks][v], acc[v]);
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int kf = 16 * ft + acc_row<double>(lg, r);
-      if (kf < 3 * Ng) {
-#pragma unroll
-        for (int v = 0; v < M; ++v) SO[el * FS + kf * M + v] = acc[v][r];
-      }
-    }
-  }
-  wave_sync();
-  if (wb.active) copy_batch<FS>(a.sbar + e0 * FS, SO, cnt, lane);
+    stage_transposed<ES, 16>(LT, a.lam + e0 * ES, cnt, lane);
+  face_weights<N, EQN>(a, LT, SO, sc, a.sbar + e0 * FS, cnt, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -346,24 +365,25 @@ __global__ void __launch_bounds__((64 * AdjFaceGroup<N, EQN>::W)) adj_face_kerne
 // ---------------------------------------------------------------------------
 
 // Write the batch's transposed RHS OUT[e*ES + w] (LDS): to `out` (mode 0),
-// or the reverse LSERK4 update qbar += dt OUT, rbar *= a (mode 1).
+// or the reverse LSERK4 update qbar += dt OUT, rbar = a lam (mode 1), with
+// lam = rbar + b qbar as staged transposed in LT (LT[w*16 + e]).  With LN
+// (LDS), mode 1 also stages the next reverse stage's lam' = rbar + b' qbar
+// of the updated values, transposed the same way (zero past the batch end).
 template <int ES>
-__device__ __forceinline__ unsigned adj_write_batch(const AdjArgs &a, const double *OUT, long e0,
-                                                    int cnt, int lane) {
+__device__ __forceinline__ unsigned adj_write_batch(const AdjArgs &a, const double *OUT,
+                                                    const double *LT, double *LN, long e0, int cnt,
+                                                    int lane) {
   constexpr int NV2 = 16 * ES / 2, NIT = (NV2 + 63) / 64;
   const long base = e0 * ES;
   double *rb = (double *)a.lam;
   unsigned ef = 0;
   if (cnt == 16) {
-    dbl2 qv[NIT], rv[NIT];
+    dbl2 qv[NIT];
     if (a.mode == 1) {
 #pragma unroll
       for (int j = 0; j < NIT; ++j) {
         const int idx = lane + 64 * j;
-        if (idx < NV2) {
-          qv[j] = ((const dbl2 *)(a.qbar + base))[idx];
-          rv[j] = ((const dbl2 *)(rb + base))[idx];
-        }
+        if (idx < NV2) qv[j] = ((const dbl2 *)(a.qbar + base))[idx];
       }
     }
 #pragma unroll
@@ -375,20 +395,42 @@ __device__ __forceinline__ unsigned adj_write_batch(const AdjArgs &a, const doub
         if (a.mode == 0) {
           ((dbl2 *)(a.out + base))[idx] = v;
         } else {
-          ((dbl2 *)(a.qbar + base))[idx] = qv[j] + a.dt * v;
-          ((dbl2 *)(rb + base))[idx] = a.rk_a * rv[j];
+          dbl2 lam;
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int i = 2 * idx + h, e = i / ES, w = i - e * ES;
+            lam[h] = LT[w * 16 + e];
+          }
+          const dbl2 qn = qv[j] + a.dt * v, rn = a.rk_a * lam;
+          ((dbl2 *)(a.qbar + base))[idx] = qn;
+          ((dbl2 *)(rb + base))[idx] = rn;
+          if (LN) {
+            const dbl2 ln = rn + a.rk_b_next * qn;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              const int i = 2 * idx + h, e = i / ES, w = i - e * ES;
+              LN[w * 16 + e] = ln[h];
+            }
+          }
         }
       }
     }
   } else {
-    for (int i = lane; i < cnt * ES; i += 64) {
+    for (int i = lane; i < 16 * ES; i += 64) {
+      const int e = i / ES, w = i - e * ES;
+      if (i >= cnt * ES) {
+        if (LN) LN[w * 16 + e] = 0.0;
+        continue;
+      }
       const double val = OUT[i];
       if (val != val) ef |= EF_NAN;
       if (a.mode == 0) {
         a.out[base + i] = val;
       } else {
-        a.qbar[base + i] += a.dt * val;
-        rb[base + i] *= a.rk_a;
+        const double qn = a.qbar[base + i] + a.dt * val, rn = a.rk_a * LT[w * 16 + e];
+        a.qbar[base + i] = qn;
+        rb[base + i] = rn;
+        if (LN) LN[w * 16 + e] = rn + a.rk_b_next * qn;
       }
     }
   }
@@ -402,7 +444,13 @@ struct AdjLds {
   static constexpr int XT = Order<N>::NKS1 * 4 * M * 16;
   static constexpr int PT = 3 * Nfp * M * 16;
   static constexpr int I1D = 3 * Ng * Nfp;
-  static constexpr int doubles = 2 * XT + PT + I1D + 2 * Nfp;
+  static constexpr int MAIN = 2 * XT + PT + I1D + 2 * Nfp;
+  // epilogue: OUT (16 ES, inside XT's space), then SO (16 FS) over it; the
+  // staged lam (LT) is read while the next lam is staged past both
+  static constexpr int SO = 16 * 3 * Ng * M > 2 * XT ? 16 * 3 * Ng * M : 2 * XT;
+  static constexpr int EPI_LN = SO;
+  static constexpr int EPI = EPI_LN + XT;
+  static constexpr int doubles = MAIN > EPI ? MAIN : EPI;
 };
 
 // Waves per SIMD requested from the register allocator: two where the
@@ -453,7 +501,10 @@ __global__ void __launch_bounds__((64 * AdjGroup<N, EQN>::W), (AdjWaves<N, EQN>:
   for (int i = lane; i < L::I1D; i += 64) I1[i] = a.I1D[i];
   for (int i = lane; i < 3 * Nfp; i += 64) FM[i] = a.fmask[i];
   stage_transposed<ES, 16>(XT, a.q + e0 * ES, cnt, lane);
-  stage_transposed<ES, 16>(LT, a.lam + e0 * ES, cnt, lane);
+  if (a.mode == 1)
+    axpy_stage_transposed<ES>(LT, a.lam + e0 * ES, a.qbar + e0 * ES, a.rk_b, cnt, lane);
+  else
+    stage_transposed<ES, 16>(LT, a.lam + e0 * ES, cnt, lane);
   for (int i = 16 * ES + lane; i < L::XT; i += 64) XT[i] = LT[i] = 0.0;
   wave_sync();
   for (int it = lane; it < 16 * 3 * Nfp; it += 64) {
@@ -608,9 +659,13 @@ __global__ void __launch_bounds__((64 * AdjGroup<N, EQN>::W), (AdjWaves<N, EQN>:
       }
     }
   wave_sync();
-  if (wb.active) ef |= adj_write_batch<ES>(a, OUT, e0, cnt, lane);
+  const bool next = a.mode == 1 && a.sbar_out;
+  double *LN = lds + L::EPI_LN;
+  ef |= adj_write_batch<ES>(a, OUT, LT, next ? LN : nullptr, e0, cnt, lane);
+  // the next reverse stage's Sb of this batch (SO over OUT)
+  if (next) face_weights<N, EQN>(a, LN, lds, sc, a.sbar_out + e0 * FS, cnt, lane);
   for (int off = 32; off > 0; off >>= 1) ef |= __shfl_xor(ef, off);
-  if (lane == 0 && ef && wb.active) atomicOr(a.err, ef);
+  if (lane == 0 && ef) atomicOr(a.err, ef);
 }
 
 // ---------------------------------------------------------------------------

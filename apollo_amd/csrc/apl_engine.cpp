@@ -62,6 +62,7 @@ struct apl_handle_s {
   double *d_adjops[6] = {nullptr};   // Avc, Aart, Aast, Avct, Alt, Aift
   double *d_Vc = nullptr, *d_cub = nullptr;
   double *d_sbar = nullptr, *d_qbar = nullptr, *d_rbar = nullptr;
+  double *d_sbar2 = nullptr;         // the reverse stepper's second face-weight buffer
   double *d_stg = nullptr;           // 6 stage buffers
   double *d_seg = nullptr;           // states of one checkpoint segment
   long long seg_slots = 0;
@@ -485,7 +486,7 @@ static void free_handle(apl_handle *h) {
   free_graphs(h);
   void *bufs[] = {h->d_geom, h->d_vert, h->d_I1D, h->d_fcode, h->d_fmask, h->d_A1, h->d_A2,
                   h->d_A3, h->d_speed, h->d_t, h->d_qa, h->d_qb, h->d_res, h->d_out,
-                  h->d_ckpt, h->d_Vc, h->d_cub, h->d_sbar, h->d_qbar, h->d_rbar, h->d_stg,
+                  h->d_ckpt, h->d_Vc, h->d_cub, h->d_sbar, h->d_sbar2, h->d_qbar, h->d_rbar, h->d_stg,
                   h->d_seg, h->d_opart, h->d_lavg, h->d_lgrad, h->d_ltab, h->d_ltrc};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
@@ -1333,6 +1334,7 @@ static int ensure_adjoint_ops(apl_handle *h) {
   if ((rc = dev_upload<double>(h, &h->d_cub, cub.data(), cub.size()))) return rc;
   const size_t n = state_len(h);
   if ((rc = dev_alloc(h, (void **)&h->d_sbar, sizeof(double) * (size_t)h->K * 3 * Ng * h->M))) return rc;
+  if ((rc = dev_alloc(h, (void **)&h->d_sbar2, sizeof(double) * (size_t)h->K * 3 * Ng * h->M))) return rc;
   if ((rc = dev_alloc(h, (void **)&h->d_qbar, sizeof(double) * n))) return rc;
   if ((rc = dev_alloc(h, (void **)&h->d_rbar, sizeof(double) * n))) return rc;
   if ((rc = dev_alloc(h, (void **)&h->d_stg, sizeof(double) * n * 6))) return rc;
@@ -1404,6 +1406,7 @@ int apl_adjoint_rhs(apl_handle *h, double t, const double *q, const double *lam,
   HIPCHK(h, hipMemsetAsync(h->d_err, 0, sizeof(unsigned), h->stream));
   AdjArgs a = adj_args(h);
   a.mode = 0;
+  a.face_pass = 1;
   a.q = dq;
   a.lam = dl;
   a.out = dout;
@@ -1485,6 +1488,12 @@ int apl_lserk4_adjoint(apl_handle *h, double t0, double dt, int nsteps, int ckpt
   a.lam = h->d_rbar;
   a.qbar = h->d_qbar;
   a.dt = dt;
+  // The reverse stages run in the order (step i, stage s) = (n-1, 4), ...,
+  // (0, 0).  The first launch computes its face weights Sb in a face pass;
+  // every launch writes the next stage's Sb into the other buffer.
+  double *sb[2] = {h->d_sbar, h->d_sbar2};
+  int sbi = 0;
+  bool first = true;
   for (long long seg = (nsteps - 1) / C; seg >= 0 && nsteps > 0; --seg) {
     const int s0 = (int)(seg * C), s1 = std::min(nsteps, s0 + C);
     // states q_{s0} .. q_{s1-1} of this segment
@@ -1504,7 +1513,13 @@ int apl_lserk4_adjoint(apl_handle *h, double t0, double dt, int nsteps, int ckpt
         a.t = t + rk4c[s] * dt;
         a.rk_a = rk4a[s];
         a.rk_b = rk4b[s];
+        a.rk_b_next = s > 0 ? rk4b[s - 1] : rk4b[4];
+        a.face_pass = first ? 1 : 0;
+        a.sbar = sb[sbi];
+        a.sbar_out = (i == 0 && s == 0) ? nullptr : sb[sbi ^ 1];
         if ((rc = dispatch_adjoint(h, a))) return rc;
+        sbi ^= 1;
+        first = false;
       }
     }
   }

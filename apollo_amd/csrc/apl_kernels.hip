@@ -101,9 +101,11 @@ template int launch_rhs<APL_ORDER>(int, int, int, int, hipStream_t, const RhsArg
 
 template <int N, int EQN, int FLUX>
 static int launch_adjoint_typed(hipStream_t s, const AdjArgs &a, int nblocks) {
-  const int rc = launch_batches<AdjFaceLds<N, EQN>::doubles, AdjFaceGroup<N, EQN>::W>(
-      adj_face_kernel<N, EQN>, s, a, nblocks);
-  if (rc) return rc;
+  if (a.face_pass) {
+    const int rc = launch_batches<AdjFaceLds<N, EQN>::doubles, AdjFaceGroup<N, EQN>::W>(
+        adj_face_kernel<N, EQN>, s, a, nblocks);
+    if (rc) return rc;
+  }
   return launch_batches<AdjLds<N, EQN>::doubles, AdjGroup<N, EQN>::W>(adj_rhs_kernel<N, EQN, FLUX>, s, a,
                                                                        nblocks);
 }
