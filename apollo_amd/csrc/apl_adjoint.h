@@ -369,7 +369,7 @@ __global__ void __launch_bounds__((64 * AdjFaceGroup<N, EQN>::W)) adj_face_kerne
 // lam = rbar + b qbar as staged transposed in LT (LT[w*16 + e]).  With LN
 // (LDS), mode 1 also stages the next reverse stage's lam' = rbar + b' qbar
 // of the updated values, transposed the same way (zero past the batch end).
-template <int ES>
+template <int ES, bool STAGE>
 __device__ __forceinline__ unsigned adj_write_batch(const AdjArgs &a, const double *OUT,
                                                     const double *LT, double *LN, long e0, int cnt,
                                                     int lane) {
@@ -379,7 +379,7 @@ __device__ __forceinline__ unsigned adj_write_batch(const AdjArgs &a, const doub
   unsigned ef = 0;
   if (cnt == 16) {
     dbl2 qv[NIT];
-    if (a.mode == 1) {
+    if constexpr (STAGE) {
 #pragma unroll
       for (int j = 0; j < NIT; ++j) {
         const int idx = lane + 64 * j;
@@ -392,7 +392,7 @@ __device__ __forceinline__ unsigned adj_write_batch(const AdjArgs &a, const doub
       if (idx < NV2) {
         const dbl2 v = dbl2{OUT[2 * idx], OUT[2 * idx + 1]};
         if (v[0] != v[0] || v[1] != v[1]) ef |= EF_NAN;
-        if (a.mode == 0) {
+        if constexpr (!STAGE) {
           ((dbl2 *)(a.out + base))[idx] = v;
         } else {
           dbl2 lam;
@@ -424,7 +424,7 @@ __device__ __forceinline__ unsigned adj_write_batch(const AdjArgs &a, const doub
       }
       const double val = OUT[i];
       if (val != val) ef |= EF_NAN;
-      if (a.mode == 0) {
+      if constexpr (!STAGE) {
         a.out[base + i] = val;
       } else {
         const double qn = a.qbar[base + i] + a.dt * val, rn = a.rk_a * LT[w * 16 + e];
@@ -467,7 +467,10 @@ struct AdjGroup {
   static constexpr int W = group_waves(AdjLds<N, EQN>::doubles, 4 * AdjWaves<N, EQN>::value);
 };
 
-template <int N, int EQN, int FLUX>
+// STAGE: the reverse-LSERK4 stage kernel (mode 1: lam = rbar + b qbar, the
+// stage update and the next stage's face weights); otherwise out = J^T lam
+// (mode 0).
+template <int N, int EQN, int FLUX, bool STAGE>
 __global__ void __launch_bounds__((64 * AdjGroup<N, EQN>::W), (AdjWaves<N, EQN>::value))
     adj_rhs_kernel(AdjArgs a) {
   using O = Order<N>;
@@ -501,7 +504,7 @@ __global__ void __launch_bounds__((64 * AdjGroup<N, EQN>::W), (AdjWaves<N, EQN>:
   for (int i = lane; i < L::I1D; i += 64) I1[i] = a.I1D[i];
   for (int i = lane; i < 3 * Nfp; i += 64) FM[i] = a.fmask[i];
   stage_transposed<ES, 16>(XT, a.q + e0 * ES, cnt, lane);
-  if (a.mode == 1)
+  if constexpr (STAGE)
     axpy_stage_transposed<ES>(LT, a.lam + e0 * ES, a.qbar + e0 * ES, a.rk_b, cnt, lane);
   else
     stage_transposed<ES, 16>(LT, a.lam + e0 * ES, cnt, lane);
@@ -659,11 +662,15 @@ __global__ void __launch_bounds__((64 * AdjGroup<N, EQN>::W), (AdjWaves<N, EQN>:
       }
     }
   wave_sync();
-  const bool next = a.mode == 1 && a.sbar_out;
-  double *LN = lds + L::EPI_LN;
-  ef |= adj_write_batch<ES>(a, OUT, LT, next ? LN : nullptr, e0, cnt, lane);
-  // the next reverse stage's Sb of this batch (SO over OUT)
-  if (next) face_weights<N, EQN>(a, LN, lds, sc, a.sbar_out + e0 * FS, cnt, lane);
+  if constexpr (STAGE) {
+    const bool next = a.sbar_out != nullptr;
+    double *LN = lds + L::EPI_LN;
+    ef |= adj_write_batch<ES, true>(a, OUT, LT, next ? LN : nullptr, e0, cnt, lane);
+    // the next reverse stage's Sb of this batch (SO over OUT)
+    if (next) face_weights<N, EQN>(a, LN, lds, sc, a.sbar_out + e0 * FS, cnt, lane);
+  } else {
+    ef |= adj_write_batch<ES, false>(a, OUT, LT, nullptr, e0, cnt, lane);
+  }
   for (int off = 32; off > 0; off >>= 1) ef |= __shfl_xor(ef, off);
   if (lane == 0 && ef) atomicOr(a.err, ef);
 }

@@ -106,8 +106,11 @@ static int launch_adjoint_typed(hipStream_t s, const AdjArgs &a, int nblocks) {
         adj_face_kernel<N, EQN>, s, a, nblocks);
     if (rc) return rc;
   }
-  return launch_batches<AdjLds<N, EQN>::doubles, AdjGroup<N, EQN>::W>(adj_rhs_kernel<N, EQN, FLUX>, s, a,
-                                                                       nblocks);
+  if (a.mode == 1)
+    return launch_batches<AdjLds<N, EQN>::doubles, AdjGroup<N, EQN>::W>(
+        adj_rhs_kernel<N, EQN, FLUX, true>, s, a, nblocks);
+  return launch_batches<AdjLds<N, EQN>::doubles, AdjGroup<N, EQN>::W>(
+      adj_rhs_kernel<N, EQN, FLUX, false>, s, a, nblocks);
 }
 
 template <int N>

@@ -105,3 +105,32 @@ def test_explosion_first_stage_components(N):
               % (nsteps, rel_l2(q - q0, from_oracle(eqn, qo) - q0)), flush=True)
     eng.close()
     assert m_lim <= 1e-13 and e_rhs_x <= 1e-12 and abs(dt_e - dt_o) <= 1e-13 * dt_o
+
+
+@pytest.mark.parametrize("name,engine_status", [("bfs", 9), ("explosion", 8)])
+def test_n3_discontinuous_starts_fail_like_the_oracle(name, engine_status):
+    """At N = 3 the backward step's and the explosion's discontinuous starts
+    drive the oracle's pressure negative within four steps (its status 9,
+    Apollo's exit at wxeulereqn.cc:851).  The engine stops on the same runs.
+    Its flags are gathered over a whole RHS and reported density first
+    (include/apollo_dg.h), so on the explosion, where both floors are crossed
+    in the same evaluation, it returns APL_ERR_NEG_DENSITY (8) where the
+    oracle, stopping at its first failing node, reports the pressure."""
+    case = deck_case(name, 3)
+    q0 = deck_state(name, case)
+    cfl = 1.0 / (2 * (3 + 1) - 1)
+    qo0 = to_oracle(case.params.eqn, q0)
+    dt0 = O.initial_dt(case.omesh, case.ophys, qo0, cfl)
+    t1 = 3.5 * dt0
+    with pytest.raises(ValueError) as eo:
+        O.ssp_advance(case.omesh, case.ophys, 0.0, t1, dt0, cfl, qo0)
+    oracle_status = int(str(eo.value).rsplit(":", 1)[1])
+    eng = E.Engine(case.mesh, case.params)
+    q = q0.copy()
+    with pytest.raises(E.ApolloError) as ee:
+        eng.ssp_advance(0.0, t1, dt0, cfl, q)
+    eng.close()
+    print("%s N=3: oracle status %d, engine status %d" % (name, oracle_status, ee.value.status),
+          flush=True)
+    assert oracle_status == 9
+    assert ee.value.status == engine_status
