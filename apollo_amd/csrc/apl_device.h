@@ -1182,7 +1182,9 @@ __global__ void __launch_bounds__((64 * RhsGroup<N, EQN, T>::W), (MinWaves<N, EQ
           split_bf16(phr[v], gh[v], gl[v], r);
           split_bf16(phs[v], gh[v], gl[v], 4 + r);
         }
-      } else {
+      } else if (!IsF64<T>::value || 16 * cc + 4 * r < Nc) {
+        // FP64 holds points 4r..4r+3 of the chunk in step r: a step of the
+        // last chunk with no point left adds only zeros and is skipped
 #pragma unroll
         for (int rt = 0; rt < NRT; ++rt)
 #pragma unroll
