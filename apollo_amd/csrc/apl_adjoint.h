@@ -616,19 +616,16 @@ __global__ void __launch_bounds__((64 * AdjGroup<N, EQN>::W), (AdjWaves<N, EQN>:
         for (int i = 0; i < Nfp; ++i)
 #pragma unroll
           for (int v = 0; v < M; ++v) qP[v] += In[Nfp - 1 - i] * PT[((f * Nfp + i) * M + v) * 16 + el];
-        num_flux_vjp<EQN, FLUX>(a.ph, nx, ny, qM, qP, nb, qMb, qPb);
-        // the neighbour's flux at its Gauss point Ng-1-g: its interior state
-        // is qP, its exterior state this element's trace qM
-        const double *gn = a.geom + nbr * G_WORDS;
-        const double nnx = gn[G_NX + nf], nny = gn[G_NY + nf], snj = gn[G_SJ + nf];
+        // The neighbour's flux at its Gauss point Ng-1-g has this trace qM as
+        // its exterior state and the opposite normal; the fluxes the adjoint
+        // covers (LF, TM upwind) are conservative, F*(-n, qP, qM) =
+        // -F*(n, qM, qP), so its part is -(dF*/dqM)^T of its weights, and
+        // one product with the combined weights gives both.
+        const double snj = a.geom[nbr * G_WORDS + G_SJ + nf];
         const double *sn = a.sbar + nbr * FS + (nf * Ng + (Ng - 1 - g)) * M;
-        double nbn[M], dump[M];
 #pragma unroll
-        for (int v = 0; v < M; ++v) {
-          nbn[v] = sn[v] * snj;
-          dump[v] = 0.0;
-        }
-        num_flux_vjp<EQN, FLUX>(a.ph, nnx, nny, qP, qM, nbn, dump, qMb);
+        for (int v = 0; v < M; ++v) nb[v] -= sn[v] * snj;
+        num_flux_vjp<EQN, FLUX>(a.ph, nx, ny, qM, qP, nb, qMb, qPb);
       } else {
         const double *vk = a.vert + ge * 6;
         const int va = f, vb = (f + 1) % 3;

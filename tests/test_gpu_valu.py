@@ -169,3 +169,16 @@ def test_valu_error_flags_and_unsupported():
         with pytest.raises(E.ApolloError) as ei:
             E.Engine(c.mesh, _params(c, VALU, prec))
         assert ei.value.status == 11   # APL_ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("n,periodic", [(2, False), (3, True), (3, False), (8, True)])
+def test_valu_small_meshes_fp64(n, periodic):
+    """A single partial workgroup (K = 8, 18; periodic meshes start at
+    n = 3) and exactly two full ones (K = 128), FP64 against the oracle and
+    the MFMA kernel."""
+    for N in (1, 3):
+        case = euler_case(n=n, N=N, m=5, periodic=periodic)
+        q = case.state()
+        got = _rhs(case, q, VALU, 0.2)
+        fp64_parity(case, got, q, 0.2)
+        assert rel_l2(got, _rhs(case, q, MFMA, 0.2)) <= 1e-13
