@@ -1405,11 +1405,13 @@ __global__ void __launch_bounds__((64 * LinGroup<N, EQN, T>::W), (LinWaves<N, EQ
     for (int nf = 0; nf < 3; ++nf) fnb[j][nf] = a.fmask[nf * Nfp + (Nfp - 1 - i)];
   }
 
-  // Neighbour face-node values (zero on boundary faces).  Up to 5 variables
-  // they are all loaded into registers before the batch staging, so their
-  // latency overlaps it; phMaxwell loads them per k-step.
+  // Neighbour face-node values (zero on boundary faces).  FP32 and BF16x3 up
+  // to 5 variables load them all into registers before the batch staging,
+  // so their latency overlaps it; FP64 and phMaxwell load them per k-step
+  // (FP64, measured: K = 131072, N = 5 80.7 -> 77.5 us per stage;
+  // K = 294912, N = 8 455.7 -> 441.9 us).
   const double *qg = a.q;
-  constexpr bool PRE = M <= 5;
+  constexpr bool PRE = M <= 5 && sizeof(T) != 8;
   double qP[PRE ? NKP : 1][M];
   if constexpr (PRE) {
 #pragma unroll
