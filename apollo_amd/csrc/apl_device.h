@@ -1409,7 +1409,8 @@ __global__ void __launch_bounds__((64 * LinGroup<N, EQN, T>::W), (LinWaves<N, EQ
   // to 5 variables load them all into registers before the batch staging,
   // so their latency overlaps it; FP64 and phMaxwell load them per k-step
   // (FP64, measured: K = 131072, N = 5 80.7 -> 77.5 us per stage;
-  // K = 294912, N = 8 455.7 -> 441.9 us).
+  // K = 294912, N = 8 455.7 -> 441.9 us; FP32 without the early gather:
+  // N = 5 66.3 -> 68.2 us, N = 8 364 -> 388 us).
   const double *qg = a.q;
   constexpr bool PRE = M <= 5 && sizeof(T) != 8;
   double qP[PRE ? NKP : 1][M];
