@@ -825,8 +825,9 @@ __device__ __forceinline__ void face_flux_cub(const RhsArgs &a, int ks, S *fsT, 
   constexpr int Ng = Order<N>::Ng, Nfp = Order<N>::Nfp;
   const int kf = 4 * ks + F.lg;
   // FP32 face flux for the FP32 and split-BF16x3 Euler LF kernels up to
-  // N = 5 (C3: 444 -> 402 us per stage; at N = 8 it takes the single-RHS
-  // error past 2e-5, and there the volume dominates)
+  // N = 5 (2D Euler vortex, K = 524288, N = 4: 444 -> 402 us per stage; at
+  // N = 8 it takes the single-RHS error past 2e-5, and there the volume
+  // dominates)
   constexpr bool F32 = sizeof(S) == 4 && FLUX == APL_FLUX_LF && N <= 5 &&
                        (EQN == APL_EQN_EULER5 || EQN == APL_EQN_EULER4);
   if constexpr (F32) {
@@ -964,8 +965,8 @@ __device__ __forceinline__ void split_state(const XE *XT, int lg, int el, bf8 (&
 //    above N = 4 and for N >= 7, where two waves spill (Euler N = 5
 //    346 -> 373, N = 7 637 -> 1380).
 //  - FP32 2D Euler (4 variables) at N = 4: four (127 VGPRs with the FP32
-//    face flux; C3 402 -> 380 us; at N = 3 four measured slower, C1 17.2 ->
-//    18.6 us; the 5-variable kernel would spill at four).
+//    face flux; K = 524288: 402 -> 380 us; at N = 3 four measured slower,
+//    K = 8192: 17.2 -> 18.6 us; the 5-variable kernel would spill at four).
 template <int N, int EQN, typename T>
 struct MinWaves {
   static constexpr bool TM = EQN == APL_EQN_MAXWELL_TM3;

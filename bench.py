@@ -469,7 +469,7 @@ def main():
         calibrate_copy(args.calibrate_copy)
         return
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (the multi-GPU design is "
+        print("bench.py: the engine runs on one GPU only (the multi-GPU design is "
               "documented, not built); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch

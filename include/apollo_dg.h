@@ -41,7 +41,7 @@ enum apl_status {
   APL_ERR_NEG_DENSITY = 8,  /* rho <= 0 in the Euler flux (wxeulereqn.cc:838) */
   APL_ERR_NEG_PRESSURE = 9, /* p <= 0 in the Euler flux (wxeulereqn.cc:851)   */
   APL_ERR_ALLOC = 10,       /* host or device allocation failed               */
-  APL_ERR_UNSUPPORTED = 11  /* valid request this build does not provide      */
+  APL_ERR_UNSUPPORTED = 11  /* valid request the library does not provide     */
 };
 /* The kernels raise flags and the host reads them once per call (once per
  * step in the steppers).  When one call raises several, the status is the
