@@ -315,8 +315,13 @@ __global__ void __launch_bounds__(64 * W, (ValuWaves<N, EQN, S>::value))
 #pragma unroll
       for (int n = 0; n < Np; ++n) {
         const S ar = vr[Np + n], as = vr[2 * Np + n];
+        // two separate multiply-adds per entry (one expression would cost a
+        // multiply, a multiply-add and an add)
 #pragma unroll
-        for (int v = 0; v < M; ++v) R[n][v] += ar * phr[v] + as * phs[v];
+        for (int v = 0; v < M; ++v) {
+          R[n][v] += ar * phr[v];
+          R[n][v] += as * phs[v];
+        }
       }
     }
   }
