@@ -1550,9 +1550,18 @@ __global__ void __launch_bounds__((64 * LinGroup<N, EQN, T>::W), (LinWaves<N, EQ
 #pragma unroll
         for (int v = 0; v < M; ++v) pv[v] = p[v][r];
         phys_flux<EQN, S>(a.ph, pv, F, G);
+        if constexpr (sizeof(S) == 8) {
+          // two multiply-adds per entry
 #pragma unroll
-        for (int v = 0; v < M; ++v)
-          acc[rt][v][r] += S(cx * double(F[v]) + cy * double(G[v]));
+          for (int v = 0; v < M; ++v) {
+            acc[rt][v][r] += cx * F[v];
+            acc[rt][v][r] += cy * G[v];
+          }
+        } else {
+#pragma unroll
+          for (int v = 0; v < M; ++v)
+            acc[rt][v][r] += S(cx * double(F[v]) + cy * double(G[v]));
+        }
       }
     }
   }
