@@ -157,6 +157,9 @@ static int dev_alloc(apl_handle *h, void **dst, size_t bytes) {
 // K = 8192 FP64 18.4 vs 20.4, FP32 15.4 vs 16.8; K = 131072 FP64 96.2 vs
 // 106.4, FP32 67.7 vs 75.2.
 constexpr bool VALU_BY_DEFAULT = true;
+// The nodal form of MAXWELL_TM3 runs on the VALU kernel under
+// APL_ALGO_AUTO (FP64 and FP32).
+constexpr bool NODAL_VALU_BY_DEFAULT = false;
 
 static int nvars(int eqn) {
   switch (eqn) {
@@ -171,8 +174,9 @@ static int nvars(int eqn) {
 static int dispatch_rhs(apl_handle *h, const RhsArgs &a) {
   const int nb = (h->K + 15) / 16;
   const apl_params &p = h->params;
-  // kernel: 0 cubature form on MFMA, 1 nodal form, 2 cubature form on VALU
-  const int lin = h->linear ? 1 : (h->valu ? 2 : 0);
+  // kernel: 0 cubature form on MFMA, 1 nodal form on MFMA, 2 cubature form
+  // on VALU, 3 nodal form on VALU
+  const int lin = h->linear ? (h->valu ? 3 : 1) : (h->valu ? 2 : 0);
   int rc;
   switch (p.order) {
     case 1: rc = launch_rhs<1>(p.eqn, p.flux, p.precision, lin, h->stream, a, nb); break;
@@ -316,15 +320,14 @@ static void make_valu_tables(const Operators &o, std::vector<T> &A1, std::vector
     for (int n = 0; n < Np; ++n) A3[(size_t)k * Np + n] = T(-L[(size_t)n * 3 * Ng + k]);
 }
 
-// Fragments of the nodal linear-flux kernel: A1 = [Dt_r; Dt_s] with
-// Dt_h = A_h V_c, A3 = -Lt with Lt = L blkdiag(I1D_f) (see apl_device.h).
-template <typename T>
-static void make_linear_fragments(const Operators &o, std::vector<T> &A1, std::vector<T> &A3) {
+// The folded operators of the nodal linear-flux form: D0 = Dt_r = A_r V_c,
+// D1 = Dt_s [Np x Np] and Lt = L blkdiag(I1D_f) [Np x 3Nfp] (see apl_device.h).
+static void fold_linear_operators(const Operators &o, std::vector<double> (&D)[2],
+                                  std::vector<double> &Lt) {
   const int Np = o.Np, Nc = o.Nc, Ng = o.Ng, Nfp = o.Nfp;
-  const int NKS1 = (Np + 3) / 4, NRT = (Np + 15) / 16, NKP = (3 * Nfp + 3) / 4;
-  std::vector<double> D[2] = {std::vector<double>((size_t)Np * Np, 0.0),
-                              std::vector<double>((size_t)Np * Np, 0.0)};
-  std::vector<double> Lt((size_t)Np * 3 * Nfp, 0.0);
+  D[0].assign((size_t)Np * Np, 0.0);
+  D[1].assign((size_t)Np * Np, 0.0);
+  Lt.assign((size_t)Np * 3 * Nfp, 0.0);
   for (int n = 0; n < Np; ++n) {
     // rows of A_r, A_s (V V^T Dr_c^T diag(w)) and L (V V^T I_f^T diag(gw))
     std::vector<double> ar(Nc), as(Nc), lr(3 * Ng);
@@ -359,6 +362,31 @@ static void make_linear_fragments(const Operators &o, std::vector<T> &A1, std::v
         Lt[(size_t)n * 3 * Nfp + f * Nfp + i] = sl;
       }
   }
+}
+
+// Tables of the nodal VALU kernel (apl_valu.h): A1 = [Dt_r; Dt_s] and
+// A3 = -Lt, row-major.
+template <typename T>
+static void make_valu_linear_tables(const Operators &o, std::vector<T> &A1, std::vector<T> &A3) {
+  std::vector<double> D[2], Lt;
+  fold_linear_operators(o, D, Lt);
+  A1.resize(2 * D[0].size());
+  for (size_t i = 0; i < D[0].size(); ++i) {
+    A1[i] = T(D[0][i]);
+    A1[D[0].size() + i] = T(D[1][i]);
+  }
+  A3.resize(Lt.size());
+  for (size_t i = 0; i < Lt.size(); ++i) A3[i] = T(-Lt[i]);
+}
+
+// Fragments of the nodal linear-flux kernel: A1 = [Dt_r; Dt_s] with
+// Dt_h = A_h V_c, A3 = -Lt with Lt = L blkdiag(I1D_f) (see apl_device.h).
+template <typename T>
+static void make_linear_fragments(const Operators &o, std::vector<T> &A1, std::vector<T> &A3) {
+  const int Np = o.Np, Nfp = o.Nfp;
+  const int NKS1 = (Np + 3) / 4, NRT = (Np + 15) / 16, NKP = (3 * Nfp + 3) / 4;
+  std::vector<double> D[2], Lt;
+  fold_linear_operators(o, D, Lt);
   A1.assign((size_t)2 * NRT * NKS1 * 64, T(0));
   for (int hh = 0; hh < 2; ++hh)
     for (int rt = 0; rt < NRT; ++rt)
@@ -380,7 +408,10 @@ static void make_linear_fragments(const Operators &o, std::vector<T> &A1, std::v
 template <typename T>
 static int upload_fragments(apl_handle *h) {
   std::vector<T> A1, A2, A3;
-  if (h->linear) {
+  if (h->linear && h->valu) {
+    make_valu_linear_tables<T>(h->ops, A1, A3);
+    A2.assign(1, T(0));
+  } else if (h->linear) {
     make_linear_fragments<T>(h->ops, A1, A3);
     A2.assign(1, T(0));
   } else if (h->valu) {
@@ -586,8 +617,16 @@ int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out)
   if (params->precision != APL_PREC_FP64 && params->precision != APL_PREC_FP32_MFMA &&
       params->precision != APL_PREC_BF16X3)
     return fail(nullptr, APL_ERR_ARG, "unknown precision");
-  if (params->algorithm < APL_ALGO_AUTO || params->algorithm > APL_ALGO_CUBATURE_VALU)
+  if (params->algorithm < APL_ALGO_AUTO || params->algorithm > APL_ALGO_NODAL_VALU)
     return fail(nullptr, APL_ERR_ARG, "unknown algorithm");
+  const bool maxwell = params->eqn == APL_EQN_MAXWELL_TM3 || params->eqn == APL_EQN_PHMAXWELL8;
+  if ((params->algorithm == APL_ALGO_NODAL_MFMA || params->algorithm == APL_ALGO_NODAL_VALU) &&
+      !maxwell)
+    return fail(nullptr, APL_ERR_ARG, "the nodal form needs a linear flux (Maxwell)");
+  if (params->algorithm == APL_ALGO_NODAL_VALU &&
+      (params->eqn != APL_EQN_MAXWELL_TM3 || params->precision == APL_PREC_BF16X3))
+    return fail(nullptr, APL_ERR_UNSUPPORTED,
+                "the VALU nodal kernel covers MAXWELL_TM3 in FP64 and FP32");
   if (params->algorithm == APL_ALGO_CUBATURE_VALU &&
       (params->order > 3 || params->precision == APL_PREC_BF16X3))
     return fail(nullptr, APL_ERR_UNSUPPORTED,
@@ -617,11 +656,17 @@ int apl_create(const apl_mesh *mesh, const apl_params *params, apl_handle **out)
 
   apl_handle *h = new apl_handle();
   h->params = *params;
-  h->linear = params->algorithm == APL_ALGO_AUTO &&
-              (params->eqn == APL_EQN_MAXWELL_TM3 || params->eqn == APL_EQN_PHMAXWELL8);
-  h->valu = !h->linear && params->order <= 3 && params->precision != APL_PREC_BF16X3 &&
-            (params->algorithm == APL_ALGO_CUBATURE_VALU ||
-             (params->algorithm != APL_ALGO_CUBATURE_MFMA && VALU_BY_DEFAULT));
+  h->linear = maxwell && (params->algorithm == APL_ALGO_AUTO ||
+                          params->algorithm == APL_ALGO_NODAL_MFMA ||
+                          params->algorithm == APL_ALGO_NODAL_VALU);
+  if (h->linear)
+    h->valu = params->algorithm == APL_ALGO_NODAL_VALU ||
+              (params->algorithm == APL_ALGO_AUTO && NODAL_VALU_BY_DEFAULT &&
+               params->eqn == APL_EQN_MAXWELL_TM3 && params->precision != APL_PREC_BF16X3);
+  else
+    h->valu = params->order <= 3 && params->precision != APL_PREC_BF16X3 &&
+              (params->algorithm == APL_ALGO_CUBATURE_VALU ||
+               (params->algorithm != APL_ALGO_CUBATURE_MFMA && VALU_BY_DEFAULT));
   h->K = mesh->K;
   h->M = M;
   h->Nv = mesh->Nv;

@@ -29,8 +29,9 @@ static int launch_batches(K kern, hipStream_t s, const A &a, int nbatches) {
   return hipGetLastError() == hipSuccess ? APL_OK : APL_ERR_HIP;
 }
 
-// kind: 0 the cubature form on MFMA, 1 the nodal form (linear fluxes),
-// 2 the cubature form on the VALU kernel (N <= 3, FP64 and FP32)
+// kind: 0 the cubature form on MFMA, 1 the nodal form (linear fluxes) on
+// MFMA, 2 the cubature form on the VALU kernel (N <= 3, FP64 and FP32),
+// 3 the nodal form on the VALU kernel (TM, FP64 and FP32)
 template <int N, int EQN, int FLUX, typename T>
 static int launch_typed(int kind, hipStream_t s, const RhsArgs &a, int nblocks) {
   if constexpr (EQN == APL_EQN_PHMAXWELL8 || EQN == APL_EQN_MAXWELL_TM3) {
@@ -38,7 +39,17 @@ static int launch_typed(int kind, hipStream_t s, const RhsArgs &a, int nblocks) 
       return launch_batches<LinLds<N, EQN, T>::doubles, LinGroup<N, EQN, T>::W>(
           rhs_linear_kernel<N, EQN, FLUX, T>, s, a, nblocks);
   }
-  if (kind == 1) return APL_ERR_ARG;
+  if constexpr (EQN == APL_EQN_MAXWELL_TM3 && sizeof(T) != 1) {
+    if (kind == 3) {
+      // 64 elements per workgroup
+      auto kern = rhs_valu_linear_kernel<N, EQN, FLUX, T>;
+      const size_t lds = sizeof(double) * (size_t)ValuLinLds<N, EQN, T>::doubles;
+      if (!allow_lds(kern, lds)) return APL_ERR_UNSUPPORTED;
+      hipLaunchKernelGGL(kern, dim3((a.K + 63) / 64), dim3(64 * ValuLinCfg<N, EQN>::W), lds, s, a);
+      return hipGetLastError() == hipSuccess ? APL_OK : APL_ERR_HIP;
+    }
+  }
+  if (kind == 1 || kind == 3) return APL_ERR_ARG;
   if (kind == 2) {
     if constexpr (N <= 3 && sizeof(T) != 1) {
       // 64 elements per workgroup of four waves, or of eight on small meshes

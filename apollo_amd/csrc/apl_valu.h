@@ -388,4 +388,252 @@ __global__ void __launch_bounds__(64 * W, (ValuWaves<N, EQN, S>::value))
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// the nodal (linear-flux) form on the vector ALU
+// ---------------------------------------------------------------------------
+//
+// The nodal form of rhs_linear_kernel (apl_device.h), one lane per element,
+// 64 elements per workgroup of W waves:
+//   surface  S(kf) = F*(q-, q+) L/2 at the 3 Nfp face nodes: wave w takes
+//            face nodes [w NFW, (w+1) NFW) and stages S in LDS;
+//   lift     R(n) = -Lt[n, :] S for its rows n in [w RPW, (w+1) RPW);
+//   volume   R(n) += jrx F(P_r) + jry G(P_r) + jsx F(P_s) + jsy G(P_s),
+//            P_r = Dt_r[n, :] Q, P_s = Dt_s[n, :] Q, for the same rows;
+// then R / J through LDS to the shared epilogue.  Operator entries are
+// wave-uniform (scalar loads); the state is read from its LDS copy once per
+// node for all of a wave's rows.  Each wave holds only its rows' residual,
+// so a wave needs few registers and many stay resident: the MFMA nodal
+// kernel's batch of 16 elements per wave is latency-bound on memory when
+// registers allow two FP64 waves per SIMD.
+
+template <int N, int EQN>
+struct ValuLinCfg {
+  using O = Order<N>;
+  static constexpr int M = Vars<EQN>::M;
+  static constexpr int Np = O::Np, Nfp = O::Nfp;
+  static constexpr int ES = Np * M, ESP = ES | 1;
+  static constexpr int NF = 3 * Nfp;                       // face nodes
+  static constexpr int W = Np >= 16 ? 8 : (Np >= 6 ? 4 : 2);   // waves per group
+  static constexpr int RPW = (Np + W - 1) / W;              // rows per wave
+  static constexpr int NFW = (NF + W - 1) / W;              // face nodes per wave
+};
+
+// LDS, in doubles: the staged states [64][ESP] (element type S), then the
+// FP64 result [64][ES] over them; the face fluxes [NF][M][64] (S); speed
+// maxima and flags.
+template <int N, int EQN, typename S>
+struct ValuLinLds {
+  using C = ValuLinCfg<N, EQN>;
+  static constexpr int QS_D = (64 * C::ESP * (int)sizeof(S) + 15) / 16 * 2;
+  static constexpr int OUT_D = 64 * C::ES;
+  static constexpr int BASE = QS_D > OUT_D ? QS_D : OUT_D;
+  static constexpr int SF_D = (C::NF * C::M * 64 * (int)sizeof(S) + 15) / 16 * 2;
+  static constexpr int doubles = BASE + SF_D + C::W + (C::W + 1) / 2;
+};
+
+// Waves per SIMD the register allocator is asked to allow: a wave holds
+// its rows' residual and their two nodal derivatives (3 RPW M values).
+template <int N, int EQN, typename S>
+struct ValuLinWaves {
+  static constexpr int value = sizeof(S) == 8 ? (N <= 5 ? 4 : (N == 6 ? 3 : 2))
+                                              : (N <= 5 ? 5 : (N == 6 ? 4 : 3));
+};
+
+template <int N, int EQN, int FLUX, typename S>
+__global__ void __launch_bounds__((64 * ValuLinCfg<N, EQN>::W), (ValuLinWaves<N, EQN, S>::value))
+    rhs_valu_linear_kernel(RhsArgs a) {
+  using C = ValuLinCfg<N, EQN>;
+  using L = ValuLinLds<N, EQN, S>;
+  constexpr int M = C::M, Np = C::Np, Nfp = C::Nfp, ES = C::ES, ESP = C::ESP, NF = C::NF;
+  constexpr int W = C::W, RPW = C::RPW, NFW = C::NFW, NT = 64 * W;
+
+  const cptr_t<S> DT = (cptr_t<S>)a.A1;        // [2][Np][Np]: Dt_r, Dt_s
+  const cptr_t<S> LT = (cptr_t<S>)a.A3;        // [Np][NF]: -Lt
+  const cptr_t<int> FMc = (cptr_t<int>)a.fmask;
+
+  const int lane = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int tid = (int)threadIdx.x;
+  const int grp = xcd_batch(blockIdx.x, gridDim.x);
+  const long e0 = (long)grp * 64;
+  const int cnt = (int)min((long)64, (long)a.K - e0);
+  const long ge = e0 + (lane < cnt ? lane : 0);
+
+  extern __shared__ double lds_all[];
+  S *QS = (S *)lds_all;                    // [64][ESP] staged states
+  double *OUT = lds_all;                   // [64][ES] result
+  S *SF = (S *)(lds_all + L::BASE);        // [NF][M][64] face fluxes
+  double *SPD = lds_all + L::BASE + L::SF_D;   // [W]
+  unsigned *EFW = (unsigned *)(SPD + W);       // [W]
+
+  // 1. the group's states (missing elements repeat element 0)
+  {
+    const double *qb = a.q + e0 * ES;
+    if (cnt == 64) {
+      constexpr int NV2 = 64 * ES / 2, NIT = (NV2 + NT - 1) / NT;
+      dbl2 buf[NIT];
+#pragma unroll
+      for (int j = 0; j < NIT; ++j) {
+        const int idx = tid + NT * j;
+        if (idx < NV2) buf[j] = ((const dbl2 *)qb)[idx];
+      }
+#pragma unroll
+      for (int j = 0; j < NIT; ++j) {
+        const int idx = tid + NT * j;
+        if (idx < NV2) {
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int i = 2 * idx + h, e = i / ES, w = i - e * ES;
+            QS[e * ESP + w] = S(buf[j][h]);
+          }
+        }
+      }
+    } else {
+      for (int i = tid; i < 64 * ES; i += NT) {
+        const int e = i / ES, w = i - e * ES;
+        QS[e * ESP + w] = S(e < cnt ? qb[i] : qb[w]);
+      }
+    }
+  }
+  __syncthreads();
+
+  const double *gk = a.geom + ge * G_WORDS;
+  double lmax = 0.0;
+  unsigned ef = 0;
+
+  // 2. surface: face nodes [wave NFW, wave NFW + NFW) of this lane's element
+#pragma unroll 1
+  for (int kf = wave * NFW; kf < (wave + 1) * NFW && kf < NF; ++kf) {
+    const int f = kf / Nfp, i = kf - f * Nfp;
+    const int code = a.fcode[ge * 3 + f];
+    const double nx = gk[G_NX + f], ny = gk[G_NY + f], sj = gk[G_SJ + f];
+    double qM[M], qP[M], fs[M];
+    const int fm = FMc[f * Nfp + i];
+#pragma unroll
+    for (int v = 0; v < M; ++v) qM[v] = double(QS[lane * ESP + fm * M + v]);
+    if (code >= 0) {
+      // own node i <-> the neighbour's node Fmask[nf][Nfp-1-i]
+      const int nf = code & 3;
+      const double *src = a.q + ((long)(code >> 2) * Np + a.fmask[nf * Nfp + (Nfp - 1 - i)]) * M;
+#pragma unroll
+      for (int v = 0; v < M; ++v) qP[v] = src[v];
+    } else {
+      boundary_state<EQN>(a.bc, -code - 1, 0.0, 0.0, 0.0, nx, ny, qM, qP);
+    }
+    double lam;
+    ef |= num_flux<EQN, FLUX>(a.ph, nx, ny, qM, qP, fs, &lam);
+    lmax = lam > lmax ? lam : lmax;
+#pragma unroll
+    for (int v = 0; v < M; ++v) SF[(kf * M + v) * 64 + lane] = S(fs[v] * sj);
+  }
+  __syncthreads();
+
+  // 3. lift and volume for rows [n0, n0 + RPW)
+  const int n0 = wave * RPW;
+  S R[RPW][M];
+#pragma unroll
+  for (int r = 0; r < RPW; ++r)
+#pragma unroll
+    for (int v = 0; v < M; ++v) R[r][v] = S(0);
+#pragma unroll 2
+  for (int kf = 0; kf < NF; ++kf) {
+    S sv[M];
+#pragma unroll
+    for (int v = 0; v < M; ++v) sv[v] = SF[(kf * M + v) * 64 + lane];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      if (n0 + r < Np) {
+        const S l = LT[(n0 + r) * NF + kf];
+#pragma unroll
+        for (int v = 0; v < M; ++v) R[r][v] += l * sv[v];
+      }
+    }
+  }
+  {
+    S pr[RPW][M], ps[RPW][M];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r)
+#pragma unroll
+      for (int v = 0; v < M; ++v) pr[r][v] = ps[r][v] = S(0);
+#pragma unroll 2
+    for (int k = 0; k < Np; ++k) {
+      S qk[M];
+#pragma unroll
+      for (int v = 0; v < M; ++v) qk[v] = QS[lane * ESP + k * M + v];
+#pragma unroll
+      for (int r = 0; r < RPW; ++r) {
+        if (n0 + r < Np) {
+          const S dr = DT[(n0 + r) * Np + k], ds = DT[Np * Np + (n0 + r) * Np + k];
+#pragma unroll
+          for (int v = 0; v < M; ++v) {
+            pr[r][v] += dr * qk[v];
+            ps[r][v] += ds * qk[v];
+          }
+        }
+      }
+    }
+    const double jrx = gk[G_JRX], jry = gk[G_JRY], jsx = gk[G_JSX], jsy = gk[G_JSY];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      S F[M], G[M];
+      phys_flux<EQN, S>(a.ph, pr[r], F, G);
+#pragma unroll
+      for (int v = 0; v < M; ++v) {
+        R[r][v] += S(jrx) * F[v];
+        R[r][v] += S(jry) * G[v];
+      }
+      phys_flux<EQN, S>(a.ph, ps[r], F, G);
+#pragma unroll
+      for (int v = 0; v < M; ++v) {
+        R[r][v] += S(jsx) * F[v];
+        R[r][v] += S(jsy) * G[v];
+      }
+    }
+  }
+
+  // 4. R / J into LDS over the staged states, then the shared epilogue: wave
+  // w writes the 16-element batches w, w + W, ...
+  __syncthreads();   // every wave has read the states
+  const double invJ = gk[G_INVJ];
+#pragma unroll
+  for (int r = 0; r < RPW; ++r)
+    if (n0 + r < Np) {
+#pragma unroll
+      for (int v = 0; v < M; ++v) OUT[lane * ES + (n0 + r) * M + v] = double(R[r][v]) * invJ;
+    }
+  __syncthreads();
+#pragma unroll
+  for (int sb = wave; sb < 4; sb += W) {
+    const long eb = e0 + 16 * sb;
+    if (eb < a.K) {
+      const int cw = (int)min((long)16, (long)a.K - eb);
+      ef |= write_batch<ES>(a, OUT + 16 * sb * ES, eb, cw, lane);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(lmax, off);
+    lmax = o > lmax ? o : lmax;
+    ef |= __shfl_xor(ef, off);
+  }
+  if (lane == 0) {
+    SPD[wave] = lmax;
+    EFW[wave] = ef;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    double m = SPD[0];
+    unsigned e = EFW[0];
+#pragma unroll
+    for (int w = 1; w < W; ++w) {
+      m = SPD[w] > m ? SPD[w] : m;
+      e |= EFW[w];
+    }
+#pragma unroll
+    for (int sb = wave; sb < 4; sb += W)
+      if (e0 + 16 * sb < a.K && a.speed_part) a.speed_part[grp * 4 + sb] = m;
+    if (wave == 0 && e) atomicOr(a.err, e);
+  }
+}
+
 }  // namespace apl

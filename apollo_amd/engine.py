@@ -23,7 +23,8 @@ APL_OK = 0
 EQN_EULER5, EQN_EULER4, EQN_PHMAXWELL8, EQN_MAXWELL_TM3 = 0, 1, 2, 3
 FLUX_LF, FLUX_UPWIND, FLUX_HLL, FLUX_ROE, FLUX_WAVE = 0, 1, 2, 3, 4
 PREC_FP64, PREC_FP32_MFMA, PREC_BF16X3 = 0, 1, 2
-ALGO_AUTO, ALGO_CUBATURE, ALGO_CUBATURE_MFMA, ALGO_CUBATURE_VALU = 0, 1, 2, 3
+ALGO_AUTO, ALGO_CUBATURE, ALGO_CUBATURE_MFMA, ALGO_CUBATURE_VALU, ALGO_NODAL_MFMA, ALGO_NODAL_VALU = (
+    0, 1, 2, 3, 4, 5)
 BC_NONE, BC_ISENTROPIC_VORTEX, BC_MAXWELL_TM, BC_PEC, BC_COPY = 0, 1, 2, 3, 4
 BC_EULER_WALL, BC_EULER_INFLOW, BC_EULER_DIRICHLET = 5, 6, 7
 HOST, DEVICE = 0, 1

@@ -80,7 +80,10 @@ VARIANTS = {
     # the cubature form on a named kernel: MFMA at every order, or the
     # vector-ALU kernel (N <= 3)
     "fp64_mfma": (0, 2), "fp32_mfma": (1, 2), "fp64_valu": (0, 3), "fp32_valu": (1, 3),
+    # the nodal form of a linear flux on a named kernel
+    "fp64_nmfma": (0, 4), "fp32_nmfma": (1, 4), "fp64_nvalu": (0, 5), "fp32_nvalu": (1, 5),
 }
+NODAL_ALGOS = (0, 4, 5)
 
 
 def engine_on_stream(w, precision, algorithm, torch):
@@ -127,7 +130,7 @@ def variant_figures(w, variant, r, steps):
     """DOF-updates/s, per-launch time and roofline figures of one run."""
     info = r["info"]
     K, N, Np, Nc, Ng, m = (info[k] for k in ("K", "N", "Np", "Nc", "Ng", "m"))
-    lin = w.linear and VARIANTS[variant][1] == 0
+    lin = w.linear and VARIANTS[variant][1] in NODAL_ALGOS
     kern_s = r["gpu_ms"] * 1e-3 / (5 * steps)
     d = dict(dof_updates_per_s=K * Np * m / kern_s, ms_per_step=r["gpu_ms"] / steps,
              kernel_us=kern_s * 1e6, form="nodal" if lin else "cubature",
@@ -437,7 +440,9 @@ def kernel_name(w, variant):
     N = w.order
     prec, algo = VARIANTS[variant]
     ty = {"fp64": "double", "fp32": "float", "bf16x3": "apl::Bf16x3"}[variant.split("_")[0]]
-    if w.linear and algo == 0:
+    if w.linear and algo == 5:
+        return "rhs_valu_linear_kernel<%d,%d,%d,%s>" % (N, w.eqn, w.flux, ty)
+    if w.linear and algo in NODAL_ALGOS:
         return "rhs_linear_kernel<%d,%d,%d,%s>" % (N, w.eqn, w.flux, ty)
     if N <= 3 and prec != 2 and algo != 2:
         nw = 8 if (w.K + 63) // 64 < 512 else 4
@@ -521,7 +526,7 @@ def main():
     # FP64 and FP32 ridges (9.8 and 19.7 flop/B), so its bound is HBM; the
     # cubature form is reported against its MFMA peak beside it.
     bpl = bytes_alg_stage(K, Np, m)
-    lin = w.linear and VARIANTS[head][1] == 0
+    lin = w.linear and VARIANTS[head][1] in NODAL_ALGOS
     kname = kernel_name(w, head)
     if lin:
         ach = bpl / (h["kernel_us"] * 1e-6)
@@ -543,7 +548,7 @@ def main():
     for v in out:
         if v == head:
             continue
-        lv = w.linear and VARIANTS[v][1] == 0
+        lv = w.linear and VARIANTS[v][1] in NODAL_ALGOS
         if lv:
             line["roofline_variants"][v] = {
                 "bound": "hbm", "achieved": out[v]["gbs_alg"], "peak": PEAK["hbm"] / 1e9,

@@ -98,10 +98,16 @@ enum apl_algorithm {
                               APL_ALGO_AUTO picks for it                     */
   APL_ALGO_CUBATURE_MFMA = 2, /* the cubature form on the MFMA kernel at every
                               order (padded 16x16x4 tiles at N <= 3)         */
-  APL_ALGO_CUBATURE_VALU = 3  /* the cubature form on the vector-ALU kernel
+  APL_ALGO_CUBATURE_VALU = 3, /* the cubature form on the vector-ALU kernel
                               (one lane per element, operators as scalar
                               operands); N <= 3, FP64 or FP32 only, else
                               APL_ERR_UNSUPPORTED                            */
+  APL_ALGO_NODAL_MFMA = 4, /* the nodal form of the linear fluxes on the MFMA
+                              kernel (Maxwell only, else APL_ERR_ARG)        */
+  APL_ALGO_NODAL_VALU = 5  /* the nodal form on the vector-ALU kernel (one
+                              lane per element, rows split over the waves of
+                              a workgroup); MAXWELL_TM3 in FP64 or FP32 only,
+                              else APL_ERR_UNSUPPORTED                       */
 };
 
 enum apl_bc_kind {

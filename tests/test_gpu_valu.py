@@ -182,3 +182,80 @@ def test_valu_small_meshes_fp64(n, periodic):
         got = _rhs(case, q, VALU, 0.2)
         fp64_parity(case, got, q, 0.2)
         assert rel_l2(got, _rhs(case, q, MFMA, 0.2)) <= 1e-13
+
+
+# --- the nodal form of the TM fluxes on the vector ALU ---------------------
+
+NVALU, NMFMA = E.ALGO_NODAL_VALU, E.ALGO_NODAL_MFMA
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6, 7, 8])
+def test_nodal_valu_tm_fp64(N):
+    """TM with PEC walls (upwind) and the TM boundary state (LF), K = 72 and
+    200 (partial last workgroup): FP64 against the oracle and the MFMA nodal
+    kernel."""
+    for n, bc, flux in ((6, E.BC_PEC, E.FLUX_UPWIND), (10, E.BC_MAXWELL_TM, E.FLUX_LF)):
+        case = maxwell_case(n=n, N=N, m=3, bc=bc, flux=flux)
+        q = case.state()
+        got = _rhs(case, q, NVALU)
+        fp64_parity(case, got, q)
+        assert rel_l2(got, _rhs(case, q, NMFMA)) <= 1e-13
+
+
+@pytest.mark.parametrize("N", [1, 3, 5, 8])
+def test_nodal_valu_tm_fp32(N):
+    case = maxwell_case(n=8, N=N, m=3, bc=E.BC_PEC, flux=E.FLUX_UPWIND)
+    q = case.state()
+    ref, _ = case.oracle_rhs(q)
+    e = rel_l2(_rhs(case, q, NVALU, precision=E.PREC_FP32_MFMA), ref)
+    print("nodal VALU FP32 N=%d: %.3e" % (N, e), flush=True)
+    assert e <= 2e-5
+
+
+@pytest.mark.parametrize("N,n", [(2, 90), (5, 41)])
+def test_nodal_valu_lserk4_matches_oracle(N, n):
+    """Fused LSERK4 stages (captured graph) of the VALU nodal kernel against
+    the oracle: K = 16200 (254 groups, 8 in the last) and 3362."""
+    case = maxwell_case(n=n, N=N, m=3, bc=E.BC_PEC, flux=E.FLUX_UPWIND)
+    x, y = case.omesh.x, case.omesh.y
+    q0 = E.tm_cavity_mode(x, y, 0.013, m=3)
+    q0 = np.ascontiguousarray(q0 + 1e-2 * np.random.default_rng(N).standard_normal(q0.shape))
+    dt = 0.2 / n / (2 * N + 1)
+    p = E.Params(N, E.EQN_MAXWELL_TM3, flux=E.FLUX_UPWIND, bcs=[(E.BC_PEC, [])], algorithm=NVALU)
+    eng = E.Engine(case.mesh, p)
+    q = q0.copy()
+    eng.lserk4(0.0, dt, 3, q)
+    eng.close()
+    with O.exact_geometry():
+        ref = from_oracle(case.params.eqn, O.lserk4(case.omesh, case.ophys, 0.0, dt, 3,
+                                                    to_oracle(case.params.eqn, q0)))
+    err = rel_l2(q - q0, ref - q0)
+    print("nodal VALU LSERK4 N=%d K=%d: %.3e" % (N, case.mesh.K, err), flush=True)
+    assert err <= 1e-11
+
+
+def test_nodal_valu_ssp_and_dt_match_mfma():
+    """Apollo's SSP stepping with its dt control: the same dt sequence and
+    state as the MFMA nodal kernel."""
+    case = maxwell_case(n=12, N=3, m=3, bc=E.BC_MAXWELL_TM, flux=E.FLUX_LF)
+    q0 = case.state(kind="cavity")
+    out = {}
+    for algo in (NVALU, NMFMA):
+        eng = E.Engine(case.mesh, _params(case, algo))
+        q = q0.copy()
+        dts = eng.ssp_advance(0.0, 0.05, 0.01, 0.5 / 7, q)
+        eng.close()
+        out[algo] = (q, dts)
+    assert np.array_equal(out[NVALU][1], out[NMFMA][1])
+    assert rel_l2(out[NVALU][0] - q0, out[NMFMA][0] - q0) <= 1e-12
+
+
+def test_nodal_valu_unsupported():
+    c = maxwell_case(n=4, N=3, m=8)
+    with pytest.raises(E.ApolloError) as ei:
+        E.Engine(c.mesh, _params(c, NVALU))
+    assert ei.value.status == 11   # APL_ERR_UNSUPPORTED: phMaxwell
+    e = euler_case(n=4, N=3, m=4)
+    with pytest.raises(E.ApolloError) as ei:
+        E.Engine(e.mesh, _params(e, NVALU))
+    assert ei.value.status == 1    # APL_ERR_ARG: the nodal form needs a linear flux
