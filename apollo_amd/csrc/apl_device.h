@@ -1426,7 +1426,7 @@ __global__ void __launch_bounds__((64 * LinGroup<N, EQN, T>::W), (LinWaves<N, EQ
   wave_sync();
 
   unsigned ef = 0;
-  const S *A1 = (const S *)a.A1;   // [2][NRT][NKS1][64]: Dt_r, Dt_s
+  const S *A1 = (const S *)a.A1;   // [tile][NKS1][64]: Dt_r, Dt_s (tiles below)
   const S *A3 = (const S *)a.A3;   // [NRT][NKP][64]: -Lt
   const Bf8x2 *B1 = (const Bf8x2 *)a.A1;   // BF16x3: [2][NRT][NK8][64]
   const Bf8x2 *B3 = (const Bf8x2 *)a.A3;   // BF16x3: [NRT][NKP8][64]
@@ -1517,51 +1517,68 @@ __global__ void __launch_bounds__((64 * LinGroup<N, EQN, T>::W), (LinWaves<N, EQ
           split_bf16(float(XT[((32 * kk + 4 * it + lg) * M + v) * XS + el]), bh[kk][v], bl[kk][v], it);
   }
 
+  // Operand tiles of the volume products.  FP32 and BF16x3: [d][rt], tile t
+  // = the derivative t & 1 of output rows 16 (t >> 1) ... .  FP64: Dt_r and
+  // Dt_s stacked in one operand, Dt_s from row OFF (Np rounded up to a
+  // multiple of 4), cut into 16-row tiles that may straddle the two.  An
+  // FP64 accumulator holds rows g + 4 r of its tile in lane group g; a shift
+  // by a multiple of 4 keeps each row's lane group, so element r of a
+  // stacked tile is, in every lane, element r2 of the accumulator tile rt2
+  // of one derivative d.  Fewer tiles where 2 Np is short of 32 NRT (N = 5:
+  // 3 instead of 4, 54 MFMAs instead of 72); the same products in the same
+  // order.
+  constexpr bool STK = IsF64<T>::value;
+  constexpr int OFF = (Np + 3) / 4 * 4;
+  constexpr int NVT = STK ? (OFF + Np + 15) / 16 : 2 * NRT;
 #pragma unroll
-  for (int rt = 0; rt < NRT; ++rt) {
+  for (int t = 0; t < NVT; ++t) {
+    const int tile = STK ? t : ((t & 1) * NRT + (t >> 1));
+    V4 p[M];
 #pragma unroll
-    for (int d = 0; d < 2; ++d) {
-      V4 p[M];
+    for (int v = 0; v < M; ++v) p[v] = V4{0, 0, 0, 0};
+    if constexpr (BF) {
 #pragma unroll
-      for (int v = 0; v < M; ++v) p[v] = V4{0, 0, 0, 0};
-      if constexpr (BF) {
+      for (int kk = 0; kk < NK8; ++kk) {
+        const Bf8x2 f1 = B1[(tile * NK8 + kk) * 64 + lane];
 #pragma unroll
-        for (int kk = 0; kk < NK8; ++kk) {
-          const Bf8x2 f1 = B1[((d * NRT + rt) * NK8 + kk) * 64 + lane];
+        for (int v = 0; v < M; ++v) p[v] = mma3(f1, bh[kk][v], bl[kk][v], p[v]);
+      }
+    } else {
+      S ad[NKS1];
 #pragma unroll
-          for (int v = 0; v < M; ++v) p[v] = mma3(f1, bh[kk][v], bl[kk][v], p[v]);
+      for (int ks = 0; ks < NKS1; ++ks) ad[ks] = A1[(tile * NKS1 + ks) * 64 + lane];
+#pragma unroll
+      for (int ks = 0; ks < NKS1; ++ks)
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+          const S b = KEEP_B ? b1[KEEP_B ? ks : 0][v] : S(XT[((4 * ks + lg) * M + v) * XS + el]);
+          p[v] = mma(ad[ks], b, p[v]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      // stacked rows 16 t + 4 r + g are nodes n0 + g of derivative d
+      const int s0 = 16 * t + 4 * r;
+      const int d = STK ? (s0 >= OFF ? 1 : 0) : (t & 1);
+      const int n0 = d == 1 && STK ? s0 - OFF : s0;
+      const int rt2 = STK ? n0 / 16 : (t >> 1), r2 = STK ? (n0 % 16) / 4 : r;
+      if (rt2 >= NRT) continue;   // padding rows only
+      const double cx = d == 0 ? jrx : jsx, cy = d == 0 ? jry : jsy;
+      S pv[M], F[M], G[M];
+#pragma unroll
+      for (int v = 0; v < M; ++v) pv[v] = p[v][r];
+      phys_flux<EQN, S>(a.ph, pv, F, G);
+      if constexpr (sizeof(S) == 8) {
+        // two multiply-adds per entry
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+          acc[rt2][v][r2] += cx * F[v];
+          acc[rt2][v][r2] += cy * G[v];
         }
       } else {
-        S ad[NKS1];
 #pragma unroll
-        for (int ks = 0; ks < NKS1; ++ks) ad[ks] = A1[((d * NRT + rt) * NKS1 + ks) * 64 + lane];
-#pragma unroll
-        for (int ks = 0; ks < NKS1; ++ks)
-#pragma unroll
-          for (int v = 0; v < M; ++v) {
-            const S b = KEEP_B ? b1[KEEP_B ? ks : 0][v] : S(XT[((4 * ks + lg) * M + v) * XS + el]);
-            p[v] = mma(ad[ks], b, p[v]);
-          }
-      }
-      const double cx = d == 0 ? jrx : jsx, cy = d == 0 ? jry : jsy;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        S pv[M], F[M], G[M];
-#pragma unroll
-        for (int v = 0; v < M; ++v) pv[v] = p[v][r];
-        phys_flux<EQN, S>(a.ph, pv, F, G);
-        if constexpr (sizeof(S) == 8) {
-          // two multiply-adds per entry
-#pragma unroll
-          for (int v = 0; v < M; ++v) {
-            acc[rt][v][r] += cx * F[v];
-            acc[rt][v][r] += cy * G[v];
-          }
-        } else {
-#pragma unroll
-          for (int v = 0; v < M; ++v)
-            acc[rt][v][r] += S(cx * double(F[v]) + cy * double(G[v]));
-        }
+        for (int v = 0; v < M; ++v)
+          acc[rt2][v][r2] += S(cx * double(F[v]) + cy * double(G[v]));
       }
     }
   }

@@ -381,21 +381,27 @@ static void make_valu_linear_tables(const Operators &o, std::vector<T> &A1, std:
 
 // Fragments of the nodal linear-flux kernel: A1 = [Dt_r; Dt_s] with
 // Dt_h = A_h V_c, A3 = -Lt with Lt = L blkdiag(I1D_f) (see apl_device.h).
+// A1 holds 16-row tiles [h][rt] of each derivative, or, `stacked` (the FP64
+// kernel), of one operand with Dt_r in rows 0..Np-1 and Dt_s from row Np
+// rounded up to a multiple of 4.
 template <typename T>
-static void make_linear_fragments(const Operators &o, std::vector<T> &A1, std::vector<T> &A3) {
+static void make_linear_fragments(const Operators &o, std::vector<T> &A1, std::vector<T> &A3,
+                                  bool stacked) {
   const int Np = o.Np, Nfp = o.Nfp;
   const int NKS1 = (Np + 3) / 4, NRT = (Np + 15) / 16, NKP = (3 * Nfp + 3) / 4;
+  const int OFF = (Np + 3) / 4 * 4;
+  const int NT = stacked ? (OFF + Np + 15) / 16 : 2 * NRT;
   std::vector<double> D[2], Lt;
   fold_linear_operators(o, D, Lt);
-  A1.assign((size_t)2 * NRT * NKS1 * 64, T(0));
-  for (int hh = 0; hh < 2; ++hh)
-    for (int rt = 0; rt < NRT; ++rt)
-      for (int ks = 0; ks < NKS1; ++ks)
-        for (int l = 0; l < 64; ++l) {
-          const int n = 16 * rt + (l & 15), j = 4 * ks + (l >> 4);
-          if (n < Np && j < Np)
-            A1[(((size_t)hh * NRT + rt) * NKS1 + ks) * 64 + l] = T(D[hh][(size_t)n * Np + j]);
-        }
+  A1.assign((size_t)NT * NKS1 * 64, T(0));
+  for (int t = 0; t < NT; ++t)
+    for (int ks = 0; ks < NKS1; ++ks)
+      for (int l = 0; l < 64; ++l) {
+        const int row = 16 * t + (l & 15), j = 4 * ks + (l >> 4);
+        const int hh = stacked ? (row >= OFF ? 1 : 0) : t / NRT;
+        const int n = stacked ? row - hh * OFF : 16 * (t % NRT) + (l & 15);
+        if (n < Np && j < Np) A1[((size_t)t * NKS1 + ks) * 64 + l] = T(D[hh][(size_t)n * Np + j]);
+      }
   A3.assign((size_t)NRT * NKP * 64, T(0));
   for (int rt = 0; rt < NRT; ++rt)
     for (int ks = 0; ks < NKP; ++ks)
@@ -412,7 +418,7 @@ static int upload_fragments(apl_handle *h) {
     make_valu_linear_tables<T>(h->ops, A1, A3);
     A2.assign(1, T(0));
   } else if (h->linear) {
-    make_linear_fragments<T>(h->ops, A1, A3);
+    make_linear_fragments<T>(h->ops, A1, A3, std::is_same<T, double>::value);
     A2.assign(1, T(0));
   } else if (h->valu) {
     make_valu_tables<T>(h->ops, A1, A3);
@@ -480,7 +486,7 @@ static int upload_fragments_bf16x3(apl_handle *h) {
   std::vector<double> A1, A2, A3;
   std::vector<Bf16Pair> B1, B2, B3;
   if (h->linear) {
-    make_linear_fragments<double>(o, A1, A3);
+    make_linear_fragments<double>(o, A1, A3, false);
     B1 = regroup_bf16(A1, 2 * NRT, NKS1);
     B3 = regroup_bf16(A3, NRT, (3 * Nfp + 3) / 4);
     B2.resize(1);
