@@ -1340,11 +1340,14 @@ __device__ __forceinline__ void gather_face_node(const double *q, int kf, int co
   }
 }
 
-// Waves per SIMD requested from the register allocator.  FP64 runs two
-// waves at every order: at N = 7, 8 that costs 24-33 spilled VGPRs but beats
-// one wave with 380 registers (N = 8, K = 294912: 446 against 511 us per
-// stage; N = 7: 394 against 419).  FP32 and BF16x3 at three waves for
-// N >= 7 measured no faster than two.
+// Waves per SIMD requested from the register allocator (a lower bound: the
+// FP64 kernel fits three at N = 5 and four or more below N = 5).  FP64
+// requests two at every order: at N = 7, 8 that costs a few spilled VGPRs
+// (1 and 5) but beats one wave with 380 registers (N = 8, K = 294912: 446
+// against 511 us per stage; N = 7: 394 against 419).  Requesting four at
+// N = 5 spills 14 VGPRs and runs 107 instead of 71 us per stage (K =
+// 131072).  FP32 and BF16x3 at three waves for N >= 7 measured no faster
+// than two.
 template <int N, int EQN, typename T>
 struct LinWaves {
   static constexpr int value = EQN == APL_EQN_PHMAXWELL8 ? 1
