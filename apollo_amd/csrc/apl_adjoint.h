@@ -336,7 +336,6 @@ __global__ void __launch_bounds__((64 * AdjFaceGroup<N, EQN>::W)) adj_face_kerne
   constexpr int M = Vars<EQN>::M;
   constexpr int Np = O::Np, Ng = O::Ng, NKS1 = O::NKS1, NFT = AdjOrder<N>::NFT;
   constexpr int ES = Np * M, FS = 3 * Ng * M;
-  typedef typename Acc<double>::type V4;
   constexpr int W = AdjFaceGroup<N, EQN>::W;
   const WaveBatch wb = wave_batch<W>(a.K);
   if (!wb.active) return;

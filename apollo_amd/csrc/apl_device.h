@@ -1024,7 +1024,11 @@ __global__ void __launch_bounds__((64 * RhsGroup<N, EQN, T>::W), (MinWaves<N, EQ
   const double *gk = a.geom + ge * G_WORDS;
   const double jrx = gk[G_JRX], jry = gk[G_JRY], jsx = gk[G_JSX], jsy = gk[G_JSY];
   const S jrx_s = S(jrx), jry_s = S(jry), jsx_s = S(jsx), jsy_s = S(jsy);
-  const double Jel = gk[G_J];
+  // 1/J as stored (one FP64 division per element on the host, as the
+  // vector-ALU kernels use it): multiplying by it instead of dividing each
+  // residual entry by J changes the result by at most an ulp and drops ~12
+  // FP64 instructions per entry
+  const double invJ = gk[G_INVJ];
   const double nx0 = gk[G_NX], nx1 = gk[G_NX + 1], nx2 = gk[G_NX + 2];
   const double ny0 = gk[G_NY], ny1 = gk[G_NY + 1], ny2 = gk[G_NY + 2];
   const double sj0 = gk[G_SJ], sj1 = gk[G_SJ + 1], sj2 = gk[G_SJ + 2];
@@ -1255,7 +1259,7 @@ __global__ void __launch_bounds__((64 * RhsGroup<N, EQN, T>::W), (MinWaves<N, EQ
   double lmax = F.lmax;
   ef |= F.ef;
 
-  // 6. epilogue: / J, stage through LDS as [e][n][v], coalesced write
+  // 6. epilogue: x 1/J, stage through LDS as [e][n][v], coalesced write
   wave_sync();
   double *OUT = lds;
 #pragma unroll
@@ -1265,7 +1269,7 @@ __global__ void __launch_bounds__((64 * RhsGroup<N, EQN, T>::W), (MinWaves<N, EQ
       const int n = 16 * rt + acc_row<T>(lg, r);
       if (n < Np) {
 #pragma unroll
-        for (int v = 0; v < M; ++v) OUT[el * ES + n * M + v] = double(acc[rt][v][r]) / Jel;
+        for (int v = 0; v < M; ++v) OUT[el * ES + n * M + v] = double(acc[rt][v][r]) * invJ;
       }
     }
   wave_sync();
@@ -1586,8 +1590,9 @@ __global__ void __launch_bounds__((64 * LinGroup<N, EQN, T>::W), (LinWaves<N, EQ
     }
   }
 
-  // epilogue: / J, stage through LDS as [e][n][v], coalesced write
-  const double Jel = gk[G_J];
+  // epilogue: x 1/J (the stored reciprocal, as in rhs_kernel), stage through
+  // LDS as [e][n][v], coalesced write
+  const double invJ = gk[G_INVJ];
   wave_sync();
   XE *OUT = XT;
 #pragma unroll
@@ -1597,7 +1602,7 @@ __global__ void __launch_bounds__((64 * LinGroup<N, EQN, T>::W), (LinWaves<N, EQ
       const int n = 16 * rt + acc_row<T>(lg, r);
       if (n < Np) {
 #pragma unroll
-        for (int v = 0; v < M; ++v) OUT[el * ES + n * M + v] = XE(double(acc[rt][v][r]) / Jel);
+        for (int v = 0; v < M; ++v) OUT[el * ES + n * M + v] = XE(double(acc[rt][v][r]) * invJ);
       }
     }
   wave_sync();
