@@ -63,8 +63,8 @@ struct apl_handle_s {
   double *d_Vc = nullptr, *d_cub = nullptr;
   double *d_sbar = nullptr, *d_qbar = nullptr, *d_rbar = nullptr;
   double *d_sbar2 = nullptr;         // the reverse stepper's second face-weight buffer
-  double *d_stg = nullptr;           // 6 stage buffers
-  double *d_seg = nullptr;           // states of one checkpoint segment
+  double *d_stg = nullptr;           // 2 stage buffers of forward_step
+  double *d_seg = nullptr;           // stage inputs of one checkpoint segment
   long long seg_slots = 0;
   double *d_ckpt = nullptr;
   long long ckpt_slots = 0;
@@ -1388,7 +1388,7 @@ static int ensure_adjoint_ops(apl_handle *h) {
   if ((rc = dev_alloc(h, (void **)&h->d_sbar2, sizeof(double) * (size_t)h->K * 3 * Ng * h->M))) return rc;
   if ((rc = dev_alloc(h, (void **)&h->d_qbar, sizeof(double) * n))) return rc;
   if ((rc = dev_alloc(h, (void **)&h->d_rbar, sizeof(double) * n))) return rc;
-  if ((rc = dev_alloc(h, (void **)&h->d_stg, sizeof(double) * n * 6))) return rc;
+  if ((rc = dev_alloc(h, (void **)&h->d_stg, sizeof(double) * n * 2))) return rc;
   return APL_OK;
 }
 
@@ -1467,10 +1467,9 @@ int apl_adjoint_rhs(apl_handle *h, double t, const double *q, const double *lam,
   return check_flags(h);
 }
 
-// One forward LSERK4 step from `in`; stage inputs go to stg[0..4] when
-// `stages` is set, the result to `result`.
-static int forward_step(apl_handle *h, double t, double dt, const double *in, double *result,
-                        bool stages) {
+// One forward LSERK4 step from `in` to `result`, its stage inputs
+// alternating between two buffers.
+static int forward_step(apl_handle *h, double t, double dt, const double *in, double *result) {
   const size_t n = state_len(h);
   double *stg = h->d_stg;
   HIPCHK(h, hipMemcpyAsync(stg, in, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
@@ -1480,8 +1479,30 @@ static int forward_step(apl_handle *h, double t, double dt, const double *in, do
   a.dt = dt;
   a.t = t;
   for (int s = 0; s < 5; ++s) {
-    a.q = stg + (stages ? s : (s & 1)) * n;
-    a.out = (s == 4) ? result : stg + (stages ? s + 1 : ((s + 1) & 1)) * n;
+    a.q = stg + (s & 1) * n;
+    a.out = (s == 4) ? result : stg + ((s + 1) & 1) * n;
+    a.rk_a = rk4a[s];
+    a.rk_b = rk4b[s];
+    a.tofs = rk4c[s] * dt;
+    int rc = dispatch_rhs(h, a);
+    if (rc) return rc;
+  }
+  return APL_OK;
+}
+
+// One LSERK4 step whose stage s reads stg[s n] and writes stage s + 1's
+// input to stg[(s + 1) n] (s < 4) or the new state to `result`; stg[0] holds
+// the step's input state.
+static int forward_stages(apl_handle *h, double t, double dt, double *stg, double *result) {
+  const size_t n = state_len(h);
+  RhsArgs a = base_args(h);
+  a.mode = 1;
+  a.res = h->d_res;
+  a.dt = dt;
+  a.t = t;
+  for (int s = 0; s < 5; ++s) {
+    a.q = stg + (size_t)s * n;
+    a.out = (s == 4) ? result : stg + (size_t)(s + 1) * n;
     a.rk_a = rk4a[s];
     a.rk_b = rk4b[s];
     a.tofs = rk4c[s] * dt;
@@ -1510,12 +1531,14 @@ int apl_lserk4_adjoint(apl_handle *h, double t0, double dt, int nsteps, int ckpt
     if ((rc = dev_alloc(h, (void **)&h->d_ckpt, bytes * (size_t)nck))) return rc;
     h->ckpt_slots = nck;
   }
-  if (h->seg_slots < C) {
+  // the five stage inputs of every step of one segment
+  const long long nseg = 5LL * C;
+  if (h->seg_slots < nseg) {
     if (h->d_seg) (void)hipFree(h->d_seg);
     h->d_seg = nullptr;
     h->seg_slots = 0;
-    if ((rc = dev_alloc(h, (void **)&h->d_seg, bytes * (size_t)C))) return rc;
-    h->seg_slots = C;
+    if ((rc = dev_alloc(h, (void **)&h->d_seg, bytes * (size_t)nseg))) return rc;
+    h->seg_slots = nseg;
   }
   const hipMemcpyKind in_kind = where == APL_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
   HIPCHK(h, hipMemsetAsync(h->d_err, 0, sizeof(unsigned), h->stream));
@@ -1524,7 +1547,7 @@ int apl_lserk4_adjoint(apl_handle *h, double t0, double dt, int nsteps, int ckpt
   HIPCHK(h, hipMemcpyAsync(cur, q0, bytes, in_kind, h->stream));
   for (int step = 0; step < nsteps; ++step) {
     double *dst = ((step + 1) % C == 0) ? h->d_ckpt + (size_t)((step + 1) / C) * n : h->d_qa;
-    if ((rc = forward_step(h, t0 + step * dt, dt, cur, dst, false))) return rc;
+    if ((rc = forward_step(h, t0 + step * dt, dt, cur, dst))) return rc;
     cur = dst;
   }
   if (qT) {
@@ -1547,20 +1570,20 @@ int apl_lserk4_adjoint(apl_handle *h, double t0, double dt, int nsteps, int ckpt
   bool first = true;
   for (long long seg = (nsteps - 1) / C; seg >= 0 && nsteps > 0; --seg) {
     const int s0 = (int)(seg * C), s1 = std::min(nsteps, s0 + C);
-    // states q_{s0} .. q_{s1-1} of this segment
+    // the segment recomputed from its checkpoint, keeping the stage inputs
+    // of step i in slots 5 (i - s0) .. 5 (i - s0) + 4 (slot 5 (i - s0) is
+    // q_i), so that the reverse stages need no further forward work
     HIPCHK(h, hipMemcpyAsync(h->d_seg, h->d_ckpt + (size_t)seg * n, bytes, hipMemcpyDeviceToDevice,
                              h->stream));
-    for (int i = s0 + 1; i < s1; ++i)
-      if ((rc = forward_step(h, t0 + (i - 1) * dt, dt, h->d_seg + (size_t)(i - 1 - s0) * n,
-                             h->d_seg + (size_t)(i - s0) * n, false)))
-        return rc;
+    for (int i = s0; i < s1; ++i) {
+      double *slot = h->d_seg + (size_t)5 * (i - s0) * n;
+      double *next = i + 1 < s1 ? slot + 5 * n : h->d_qa;
+      if ((rc = forward_stages(h, t0 + i * dt, dt, slot, next))) return rc;
+    }
     for (int i = s1 - 1; i >= s0; --i) {
       const double t = t0 + i * dt;
-      // stage inputs of step i in stg[0..4]
-      if ((rc = forward_step(h, t, dt, h->d_seg + (size_t)(i - s0) * n, h->d_stg + 5 * n, true)))
-        return rc;
       for (int s = 4; s >= 0; --s) {
-        a.q = h->d_stg + (size_t)s * n;
+        a.q = h->d_seg + (size_t)(5 * (i - s0) + s) * n;
         a.t = t + rk4c[s] * dt;
         a.rk_a = rk4a[s];
         a.rk_b = rk4b[s];

@@ -268,8 +268,8 @@ def run_adjoint(torch):
                        "noise(J) / (eps |g.d|) (the predicted figure) and grows as eps shrinks "
                        "(compare eps 1e-5, 1e-6, 1e-7)",
                note="adjoint DOF-updates/s counts the 5 x steps adjoint RHS evaluations of the "
-                    "reverse sweep over the whole reverse LSERK4 call (forward sweep, segment "
-                    "recompute and stage rebuild included)")
+                    "reverse sweep over the whole reverse LSERK4 call (forward sweep and the "
+                    "segment recompute, which keeps each step's stage inputs, included)")
     eng.close()
     return res
 

@@ -298,8 +298,10 @@ int apl_adjoint_rhs(apl_handle *h, double t, const double *q, const double *lam,
 
 /* Reverse-mode LSERK4: given q0 and the terminal adjoint qbarT = dJ/dq(T)
  * after nsteps fixed-dt steps from t0, returns dJdq0 = dJ/dq(0).  Forward
- * states are checkpointed every `ckpt_every` steps in device memory; stage
- * states inside a segment are recomputed.  Optionally returns q(T). */
+ * states are checkpointed every `ckpt_every` steps in device memory; each
+ * segment is recomputed once from its checkpoint, keeping the five stage
+ * inputs of each of its steps (device memory for nsteps / ckpt_every + 1 +
+ * 5 ckpt_every states).  Optionally returns q(T). */
 int apl_lserk4_adjoint(apl_handle *h, double t0, double dt, int nsteps,
                        int ckpt_every, const double *q0, const double *qbarT,
                        double *dJdq0, double *qT, int where);
