@@ -1472,14 +1472,14 @@ int apl_adjoint_rhs(apl_handle *h, double t, const double *q, const double *lam,
 static int forward_step(apl_handle *h, double t, double dt, const double *in, double *result) {
   const size_t n = state_len(h);
   double *stg = h->d_stg;
-  HIPCHK(h, hipMemcpyAsync(stg, in, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
   RhsArgs a = base_args(h);
   a.mode = 1;
   a.res = h->d_res;
   a.dt = dt;
   a.t = t;
+  // stage 0 reads `in` (which may be `result`: stage 4 writes it last)
   for (int s = 0; s < 5; ++s) {
-    a.q = stg + (s & 1) * n;
+    a.q = s == 0 ? in : stg + (s & 1) * n;
     a.out = (s == 4) ? result : stg + ((s + 1) & 1) * n;
     a.rk_a = rk4a[s];
     a.rk_b = rk4b[s];
@@ -1542,13 +1542,24 @@ int apl_lserk4_adjoint(apl_handle *h, double t0, double dt, int nsteps, int ckpt
   }
   const hipMemcpyKind in_kind = where == APL_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
   HIPCHK(h, hipMemsetAsync(h->d_err, 0, sizeof(unsigned), h->stream));
-  // forward sweep with checkpoints
+  // forward sweep with checkpoints; the steps of the last segment keep their
+  // stage inputs in d_seg at once, so that segment needs no recompute
+  const long long last = nsteps > 0 ? (nsteps - 1) / C : -1;
+  const int sl = (int)(last * C);
   double *cur = h->d_ckpt;
   HIPCHK(h, hipMemcpyAsync(cur, q0, bytes, in_kind, h->stream));
   for (int step = 0; step < nsteps; ++step) {
-    double *dst = ((step + 1) % C == 0) ? h->d_ckpt + (size_t)((step + 1) / C) * n : h->d_qa;
-    if ((rc = forward_step(h, t0 + step * dt, dt, cur, dst))) return rc;
-    cur = dst;
+    if (step >= sl) {
+      double *slot = h->d_seg + (size_t)5 * (step - sl) * n;
+      if (step == sl) HIPCHK(h, hipMemcpyAsync(slot, cur, bytes, hipMemcpyDeviceToDevice, h->stream));
+      double *next = step + 1 < nsteps ? slot + 5 * n : h->d_qa;
+      if ((rc = forward_stages(h, t0 + step * dt, dt, slot, next))) return rc;
+      cur = next;
+    } else {
+      double *dst = ((step + 1) % C == 0) ? h->d_ckpt + (size_t)((step + 1) / C) * n : h->d_qa;
+      if ((rc = forward_step(h, t0 + step * dt, dt, cur, dst))) return rc;
+      cur = dst;
+    }
   }
   if (qT) {
     const hipMemcpyKind out_kind = where == APL_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
@@ -1570,15 +1581,18 @@ int apl_lserk4_adjoint(apl_handle *h, double t0, double dt, int nsteps, int ckpt
   bool first = true;
   for (long long seg = (nsteps - 1) / C; seg >= 0 && nsteps > 0; --seg) {
     const int s0 = (int)(seg * C), s1 = std::min(nsteps, s0 + C);
-    // the segment recomputed from its checkpoint, keeping the stage inputs
-    // of step i in slots 5 (i - s0) .. 5 (i - s0) + 4 (slot 5 (i - s0) is
-    // q_i), so that the reverse stages need no further forward work
-    HIPCHK(h, hipMemcpyAsync(h->d_seg, h->d_ckpt + (size_t)seg * n, bytes, hipMemcpyDeviceToDevice,
-                             h->stream));
-    for (int i = s0; i < s1; ++i) {
-      double *slot = h->d_seg + (size_t)5 * (i - s0) * n;
-      double *next = i + 1 < s1 ? slot + 5 * n : h->d_qa;
-      if ((rc = forward_stages(h, t0 + i * dt, dt, slot, next))) return rc;
+    // the segment recomputed from its checkpoint (the last one was kept by
+    // the forward sweep), keeping the stage inputs of step i in slots
+    // 5 (i - s0) .. 5 (i - s0) + 4 (slot 5 (i - s0) is q_i), so that the
+    // reverse stages need no further forward work
+    if (seg != last) {
+      HIPCHK(h, hipMemcpyAsync(h->d_seg, h->d_ckpt + (size_t)seg * n, bytes,
+                               hipMemcpyDeviceToDevice, h->stream));
+      for (int i = s0; i < s1; ++i) {
+        double *slot = h->d_seg + (size_t)5 * (i - s0) * n;
+        double *next = i + 1 < s1 ? slot + 5 * n : h->d_qa;
+        if ((rc = forward_stages(h, t0 + i * dt, dt, slot, next))) return rc;
+      }
     }
     for (int i = s1 - 1; i >= s0; --i) {
       const double t = t0 + i * dt;
