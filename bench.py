@@ -532,7 +532,7 @@ def main():
         ach = bpl / (h["kernel_us"] * 1e-6)
         line["roofline"] = {"bound": "hbm", "achieved": ach / 1e9, "peak": PEAK["hbm"] / 1e9,
                             "unit": "GB/s", "frac": ach / PEAK["hbm"],
-                            "traffic": measured_traffic("c2/%s" % head),
+                            "traffic": measured_traffic("tm_cavity_256_N5/%s" % head),
                             "kernel": kname + " (fused RHS + LSERK4 stage), avg over %d launches"
                                       % (5 * args.steps),
                             "bytes_per_launch": bpl}
@@ -553,13 +553,13 @@ def main():
             line["roofline_variants"][v] = {
                 "bound": "hbm", "achieved": out[v]["gbs_alg"], "peak": PEAK["hbm"] / 1e9,
                 "unit": "GB/s", "frac": out[v]["hbm_frac_alg"],
-                "traffic": measured_traffic("c2/%s" % v), "kernel": kernel_name(w, v)}
+                "traffic": measured_traffic("tm_cavity_256_N5/%s" % v), "kernel": kernel_name(w, v)}
         else:
             pk = PEAK["fp64" if v.startswith("fp64") else ("fp32" if v.startswith("fp32") else "bf16")]
             line["roofline_variants"][v] = {
                 "bound": "mfma", "achieved_exec": out[v]["tflops_exec"], "peak": pk / 1e12,
                 "unit": "TFLOP/s", "frac": out[v]["tflops_exec"] * 1e12 / pk,
-                "traffic": measured_traffic("c2/%s" % v), "kernel": kernel_name(w, v)}
+                "traffic": measured_traffic("tm_cavity_256_N5/%s" % v), "kernel": kernel_name(w, v)}
     if "fp32" in out and "fp64" in out:
         line["speedup_fp32_vs_fp64"] = out["fp32"]["speedup_vs_fp64"]
     if not args.no_accuracy and not args.only:
