@@ -1526,6 +1526,7 @@ int apl_lserk4_adjoint(apl_handle *h, double t0, double dt, int nsteps, int ckpt
   const long long nck = nsteps / C + 1;
   if (h->ckpt_slots < nck) {
     if (h->d_ckpt) (void)hipFree(h->d_ckpt);
+    h->dev_bytes -= (long long)bytes * h->ckpt_slots;
     h->d_ckpt = nullptr;
     h->ckpt_slots = 0;
     if ((rc = dev_alloc(h, (void **)&h->d_ckpt, bytes * (size_t)nck))) return rc;
@@ -1535,6 +1536,7 @@ int apl_lserk4_adjoint(apl_handle *h, double t0, double dt, int nsteps, int ckpt
   const long long nseg = 5LL * C;
   if (h->seg_slots < nseg) {
     if (h->d_seg) (void)hipFree(h->d_seg);
+    h->dev_bytes -= (long long)bytes * h->seg_slots;
     h->d_seg = nullptr;
     h->seg_slots = 0;
     if ((rc = dev_alloc(h, (void **)&h->d_seg, bytes * (size_t)nseg))) return rc;
