@@ -80,6 +80,26 @@ def test_reverse_lserk4_matches_cpu(ckpt):
     assert err <= 1e-11
 
 
+def test_reverse_lserk4_same_for_every_checkpoint_interval():
+    """The checkpoint interval only decides which forward states are kept
+    and which recomputed, from the same inputs by the same kernels: the
+    gradient and q(T) are bit-identical for every interval, including
+    ragged last segments (7 steps) and an interval longer than the run."""
+    case = euler_case(n=5, N=3, m=4, periodic=False)
+    q0 = case.state()
+    gT = np.random.default_rng(5).standard_normal(q0.shape)
+    eng = E.Engine(case.mesh, case.params)
+    out = {}
+    for ckpt in (1, 2, 3, 7, 100):
+        g0, qT = np.zeros_like(q0), np.zeros_like(q0)
+        eng.lserk4_adjoint(0.2, 0.01, 7, ckpt, q0, gT, g0, qT)
+        out[ckpt] = (g0, qT)
+    eng.close()
+    for ckpt, (g0, qT) in out.items():
+        assert np.array_equal(g0, out[1][0]), ckpt
+        assert np.array_equal(qT, out[1][1]), ckpt
+
+
 def test_objective_matches_oracle():
     case = euler_case(n=8, N=3, m=4)
     q = case.state()
